@@ -1,0 +1,189 @@
+/*
+ * poyhip.h -- C ABI of the MI355X (gfx950) implementation of POY's
+ * direct-optimization pairwise alignment path.
+ *
+ * Every per-pair entry point replaces one OCaml `external` of POY 5.1's
+ * Sequence.Align / Sequence module; the comment on each names the reference
+ * stub (path:line in the POY source tree) whose arguments and results it
+ * mirrors.  The batch and serving entry points have no reference
+ * counterpart: they are the additions that run many pairs per launch.
+ *
+ * Conventions
+ *  - Sequences are byte arrays of 5-bit nucleotide bitsets (A=1 C=2 G=4 T=8
+ *    gap=16, IUPAC codes are ORs) and start with POY's leading gap (16).
+ *  - Output sequences are written left to right into caller buffers.
+ *  - Every function returns POYHIP_OK or a negative POYHIP_E* code and never
+ *    aborts; poyhip_last_error() describes the last failure of a context.
+ *  - A context owns one device stream and its scratch; calls on one context
+ *    are not reentrant (POY's global Matrix.default had the same property).
+ *  - No PyTorch or HIP types appear in signatures.  Pointers named d_* are
+ *    device pointers (memory of the context's device); all others are host.
+ */
+#ifndef POYHIP_H
+#define POYHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define POYHIP_OK 0
+#define POYHIP_EINVAL -1   /* bad argument (e.g. longer sequence first)     */
+#define POYHIP_ENOMEM -2   /* device or host allocation failed               */
+#define POYHIP_EDEVICE -3  /* HIP runtime error                              */
+#define POYHIP_ESTATE -4   /* call order violated (no cost matrix, no fill)  */
+#define POYHIP_ECAP -5     /* output capacity too small                      */
+#define POYHIP_EBUSY -6    /* serving: admission refused (memory budget)     */
+
+typedef struct poyhip_ctx poyhip_ctx;
+
+/* The fields of POY's struct cm (src/cm.h:33-76) used on the DNA path,
+ * tables indexed (a << lcm) + b with lcm = 5.  Filled by the caller, as POY's
+ * Cost_matrix.Two_D pushes its tables through cm_CAML_set_* (src/cm.c:
+ * 2345-2407). */
+typedef struct {
+  int lcm;          /* 5 */
+  int gap;          /* 16 */
+  int a_sz;         /* 31 */
+  int combinations; /* 1 */
+  int cost_model;   /* 0 linear, 1 affine (cost_model_type) */
+  int gap_open;     /* affine gap opening (cm_get_gap_opening_parameter) */
+  int cost[2048];
+  int worst[2048];
+  int prepend[64];
+  int tail[64];
+  uint8_t median[2048];
+} poyhip_cm;
+
+/* ---- context ------------------------------------------------------------ */
+/* Replaces the module initialisers (cm_CAML_initialize src/cm.c:2529,
+ * mat_CAML_initialize src/matrices.c:345): opens `device`, creates a stream
+ * and the context's scratch (the counterpart of Matrix.default). */
+int poyhip_create(int device, poyhip_ctx **out);
+int poyhip_destroy(poyhip_ctx *ctx);
+const char *poyhip_last_error(const poyhip_ctx *ctx);
+/* cm_CAML_create + cm_CAML_set_* (src/cm.c:2452, 2345-2407): installs the
+ * 2D cost model used by every later call on the context. */
+int poyhip_set_cm(poyhip_ctx *ctx, const poyhip_cm *cm);
+/* mat_CAML_flush_memory (src/matrices.c:352): frees the context scratch. */
+int poyhip_flush(poyhip_ctx *ctx);
+
+/* ---- per-pair drop-ins (one launch per pair, synchronous) --------------- */
+
+/* algn_CAML_simple_2 (src/algn.c:3134) = Align.c_cost_2
+ * (src/sequence.ml:617): non-affine cost of s1 (not longer than s2) with
+ * the band hint deltawh.  Like the reference it leaves the direction store
+ * of this fill in the context for a following poyhip_backtrace_2d. */
+int poyhip_cost_2(poyhip_ctx *ctx, const uint8_t *s1, int l1, const uint8_t *s2,
+                  int l2, int deltawh, int *cost);
+
+/* algn_CAML_backtrace_2d (src/algn.c:3424) = Align.extract_edited_2
+ * (src/sequence.ml:944): aligned s1/s2 from the last poyhip_cost_2 on the
+ * same sequences.  cap >= l1 + l2. */
+int poyhip_backtrace_2d(poyhip_ctx *ctx, const uint8_t *s1, int l1,
+                        const uint8_t *s2, int l2, uint8_t *r1, uint8_t *r2,
+                        int cap, int *len, int swaped);
+
+/* algn_CAML_align_2d (src/algn.c:3503) = Align.c_align_2: fill + backtrace. */
+int poyhip_align_2d(poyhip_ctx *ctx, const uint8_t *s1, int l1, const uint8_t *s2,
+                    int l2, int deltawh, int swaped, uint8_t *r1, uint8_t *r2,
+                    int cap, int *len, int *cost);
+
+/* algn_CAML_cost_affine_3 (src/algn.c:2457) = Align.cost_2_affine
+ * (src/sequence.ml:630): full affine DP without directions, either order. */
+int poyhip_cost_affine_3(poyhip_ctx *ctx, const uint8_t *si, int li,
+                         const uint8_t *sj, int lj, int *cost);
+
+/* algn_CAML_align_affine_3 (src/algn.c:2359) = Align.c_align_affine_3
+ * (src/sequence.ml:626): banded affine fill with T-doubling and
+ * backtrace_aff.  li <= lj (else POYHIP_EINVAL, the reference's
+ * "pass the shorter one as first").  Each output buffer holds cap >=
+ * li + lj + 2 bytes. */
+int poyhip_align_affine_3(poyhip_ctx *ctx, const uint8_t *si, int li,
+                          const uint8_t *sj, int lj, int swaped, uint8_t *median,
+                          int *median_len, uint8_t *medianwg, int *medianwg_len,
+                          uint8_t *resi, uint8_t *resj, int *res_len, int cap,
+                          int *cost);
+
+/* algn_CAML_ancestor_2 (src/algn.c:3742): median of an aligned pair with
+ * POY's gap rules; cap >= len + 1. */
+int poyhip_ancestor_2(poyhip_ctx *ctx, const uint8_t *a, const uint8_t *b, int len,
+                      uint8_t *out, int cap, int *out_len);
+/* seq_CAML_median_2_no_gaps / _with_gaps (src/seq.c:275, 288). */
+int poyhip_median_2(poyhip_ctx *ctx, const uint8_t *a, const uint8_t *b, int len,
+                    uint8_t *out, int cap, int *out_len);
+int poyhip_median_2_with_gaps(poyhip_ctx *ctx, const uint8_t *a, const uint8_t *b,
+                              int len, uint8_t *out, int cap, int *out_len);
+/* algn_CAML_worst_2 (src/algn.c:3108) behind Align.max_cost_2. */
+int poyhip_worst_2(poyhip_ctx *ctx, const uint8_t *a, const uint8_t *b, int len,
+                   int *cost);
+
+/* ---- batches (no reference counterpart) --------------------------------- */
+/* Operation of a batch. */
+#define POYHIP_OP_ALIGN_AFFINE 1 /* align_affine_3 per pair (shorter first  */
+                                 /* and swap flag handled inside)            */
+#define POYHIP_OP_COST_AFFINE 2  /* cost_2_affine per pair                   */
+#define POYHIP_OP_ALIGN_2D 3     /* Align.align_2 (linear) + ancestor_2 +    */
+                                 /* median_2_with_gaps per pair              */
+#define POYHIP_OP_COST_2D 4      /* Align.cost_2 (linear)                    */
+
+typedef struct poyhip_batch poyhip_batch;
+
+/* Plans a batch of n pairs whose lengths (leading gap included) are
+ * len1[k], len2[k].  Planning sorts pairs largest-first for the persistent
+ * kernels, sizes every per-pair region and splits the batch into chunks whose
+ * direction stores fit `dm_budget_bytes` of device memory (0 = automatic).
+ * deltaw: NULL for Align.cost_2's ?deltaw=None (what align_2 uses), else the
+ * per-pair ~deltaw of DOS.distance. */
+int poyhip_batch_create(poyhip_ctx *ctx, int op, int n, const int *len1,
+                        const int *len2, const int *deltaw, size_t dm_budget_bytes,
+                        poyhip_batch **out);
+int poyhip_batch_destroy(poyhip_batch *b);
+/* Bytes of the output block of the batch and the offset of pair k in it.
+ * Align ops write four consecutive regions of cap_k bytes per pair (affine:
+ * resi, resj, median, medianwg with cap_k = l1+l2+2; linear: s1', s2',
+ * ancestor median, median with gaps with cap_k = l1+l2+1).  Each region
+ * holds its sequence right-aligned: it starts at cap_k - length. */
+size_t poyhip_batch_out_bytes(const poyhip_batch *b);
+int64_t poyhip_batch_out_offset(const poyhip_batch *b, int k);
+int poyhip_batch_out_cap(const poyhip_batch *b, int k);
+/* Runs the batch on device-resident inputs.  d_blob holds all sequences;
+ * d_off1/d_off2 give the offsets of the caller's first and second sequence
+ * of each pair.  Results: d_cost[n]; for align ops d_out (out_bytes) and
+ * d_outlen[4n] (region lengths in the order above, caller order for
+ * the two aligned sequences).  Asynchronous on the context stream unless
+ * `sync` is non-zero. */
+int poyhip_batch_run(poyhip_batch *b, const uint8_t *d_blob, const int64_t *d_off1,
+                     const int64_t *d_off2, int *d_cost, uint8_t *d_out,
+                     int *d_outlen, int sync);
+/* Per-pair statistics of the last run (host arrays of n; any may be NULL):
+ * cells evaluated over all doubling rounds, rounds, final band half-width. */
+int poyhip_batch_stats(poyhip_batch *b, int64_t *cells, int *rounds, int *newk);
+/* Captures the batch's launches into a HIP graph and replays it on later
+ * runs with the same device pointers. */
+int poyhip_batch_use_graph(poyhip_batch *b, int enable);
+/* Average duration (ms) of the fill kernel over the last run, measured with
+ * HIP events on the context stream. */
+float poyhip_batch_last_fill_ms(const poyhip_batch *b);
+float poyhip_batch_last_total_ms(const poyhip_batch *b);
+
+/* ---- synchronisation helpers -------------------------------------------- */
+int poyhip_sync(poyhip_ctx *ctx);
+/* Opaque hipStream_t of the context (for callers that enqueue their own
+ * copies on it). */
+void *poyhip_stream(poyhip_ctx *ctx);
+
+/* ---- workloads (synthetic inputs of the benchmark configurations) ------- */
+/* SplitMix64-seeded pair generators; see the package documentation for the
+ * mutation process.  Two-pass: call with blob == NULL to get the total
+ * bytes, then again to fill.  Sequences are written back to back; off/len
+ * arrays have 2n entries (first, second sequence of each pair). */
+int poyhip_gen_pairs(int config, int n, uint64_t seed, uint8_t *blob, int64_t *off,
+                     int *len, int64_t *total_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* POYHIP_H */

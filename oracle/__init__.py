@@ -1,0 +1,272 @@
+"""CPU oracle for POY's DO pairwise alignment -- TEST INFRASTRUCTURE ONLY.
+
+ctypes binding of ``oracle/liboracle.so`` (built from ``oracle/oracle.c`` by
+``oracle/Makefile``).  Only ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` may import this package, and only as the
+checker.  The product package ``poy5_amd`` never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liboracle.so")
+
+TBL = 2048
+GAP = 16
+
+
+class OrcCM(C.Structure):
+    _fields_ = [
+        ("lcm", C.c_int), ("gap", C.c_int), ("a_sz", C.c_int),
+        ("combinations", C.c_int), ("cost_model", C.c_int), ("gap_open", C.c_int),
+        ("cost", C.c_int * TBL), ("worst", C.c_int * TBL),
+        ("prepend", C.c_int * 64), ("tail", C.c_int * 64),
+        ("median", C.c_uint8 * TBL),
+    ]
+
+
+class OrcCM3(C.Structure):
+    _fields_ = [
+        ("lcm", C.c_int), ("gap", C.c_int), ("a_sz", C.c_int),
+        ("cost", C.c_int * 32768), ("median", C.c_uint8 * 32768),
+    ]
+
+
+class OrcStats(C.Structure):
+    _fields_ = [
+        ("cells", C.c_int64), ("rounds", C.c_int), ("full_plane", C.c_int),
+        ("final_T", C.c_int), ("final_k", C.c_int), ("gap_num", C.c_int),
+    ]
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        L = C.CDLL(_LIB)
+        u8p = C.POINTER(C.c_uint8)
+        ip = C.POINTER(C.c_int)
+        u16p = C.POINTER(C.c_uint16)
+        L.orc_cm_dna.argtypes = [C.POINTER(OrcCM), C.c_int, C.c_int, C.c_int]
+        L.orc_cm3_from_2d.argtypes = [C.POINTER(OrcCM3), C.POINTER(OrcCM)]
+        L.orc_min_non0_cost.argtypes = [C.POINTER(OrcCM)]
+        L.orc_nw.argtypes = [u8p, C.c_int, u8p, C.c_int, C.POINTER(OrcCM), C.c_int,
+                             u16p, C.POINTER(OrcStats)]
+        L.orc_backtrace_2d.argtypes = [u8p, C.c_int, u8p, C.c_int, u16p, C.c_int, u8p, u8p]
+        L.orc_backtrace_2d_gaps.argtypes = [u16p, C.c_int, C.c_int]
+        for f in ("orc_ancestor_2", "orc_median_2_no_gaps", "orc_median_2_with_gaps"):
+            getattr(L, f).argtypes = [u8p, u8p, C.c_int, C.POINTER(OrcCM), u8p]
+        for f in ("orc_worst_2", "orc_verify_2"):
+            getattr(L, f).argtypes = [u8p, u8p, C.c_int, C.POINTER(OrcCM)]
+        L.orc_align_affine_3.argtypes = [u8p, C.c_int, u8p, C.c_int, C.POINTER(OrcCM), C.c_int,
+                                         u8p, ip, u8p, ip, u8p, u8p, ip, C.POINTER(OrcStats)]
+        L.orc_cost_affine_3.argtypes = [u8p, C.c_int, u8p, C.c_int, C.POINTER(OrcCM)]
+        L.orc_count_gaps.argtypes = [u8p, C.c_int]
+        L.orc_deltawh.argtypes = [u8p, C.c_int, u8p, C.c_int, C.c_int, C.c_int]
+        L.orc_cost_2.argtypes = [u8p, C.c_int, u8p, C.c_int, C.POINTER(OrcCM), C.c_int, C.c_int,
+                                 u16p, C.POINTER(OrcStats)]
+        L.orc_align_2.argtypes = [u8p, C.c_int, u8p, C.c_int, C.POINTER(OrcCM), u8p, u8p, ip,
+                                  C.POINTER(OrcStats)]
+        L.orc_nw_3d.argtypes = [u8p, C.c_int, u8p, C.c_int, u8p, C.c_int, C.POINTER(OrcCM3), u16p]
+        L.orc_backtrack_3d.argtypes = [u8p, C.c_int, u8p, C.c_int, u8p, C.c_int, u16p, C.c_int,
+                                       u8p, u8p, u8p]
+        L.orc_median_3.argtypes = [u8p, u8p, u8p, C.c_int, C.POINTER(OrcCM3), u8p]
+        L.orc_batch_align_affine.argtypes = [C.c_int, u8p, C.POINTER(C.c_int64), ip,
+                                             C.POINTER(C.c_int64), ip, C.POINTER(OrcCM), ip,
+                                             C.POINTER(C.c_int64)]
+        L.orc_batch_align_affine.restype = C.c_int64
+        _lib = L
+    return _lib
+
+
+def _u8(a):
+    a = np.ascontiguousarray(a, dtype=np.uint8)
+    return a, a.ctypes.data_as(C.POINTER(C.c_uint8))
+
+
+def cm_dna(trans: int, gaps: int, gap_open: int | None = None) -> OrcCM:
+    cm = OrcCM()
+    lib().orc_cm_dna(C.byref(cm), trans, gaps, -1 if gap_open is None else gap_open)
+    return cm
+
+
+def cm3_from_2d(cm: OrcCM) -> OrcCM3:
+    c3 = OrcCM3()
+    lib().orc_cm3_from_2d(C.byref(c3), C.byref(cm))
+    return c3
+
+
+def min_non0_cost(cm) -> int:
+    return lib().orc_min_non0_cost(C.byref(cm))
+
+
+def nw(s1, s2, cm, deltawh):
+    """algn_nw: returns (cost, dm[L1,L2] uint16, stats)."""
+    a, ap = _u8(s1)
+    b, bp = _u8(s2)
+    dm = np.zeros((len(a), len(b)), dtype=np.uint16)
+    st = OrcStats()
+    c = lib().orc_nw(ap, len(a), bp, len(b), C.byref(cm), deltawh,
+                     dm.ctypes.data_as(C.POINTER(C.c_uint16)), C.byref(st))
+    return c, dm, st
+
+
+def backtrace_2d(s1, s2, dm, swaped):
+    a, ap = _u8(s1)
+    b, bp = _u8(s2)
+    cap = len(a) + len(b)
+    r1 = np.zeros(cap, np.uint8)
+    r2 = np.zeros(cap, np.uint8)
+    n = lib().orc_backtrace_2d(ap, len(a), bp, len(b),
+                               np.ascontiguousarray(dm).ctypes.data_as(C.POINTER(C.c_uint16)),
+                               int(swaped), r1.ctypes.data_as(C.POINTER(C.c_uint8)),
+                               r2.ctypes.data_as(C.POINTER(C.c_uint8)))
+    return r1[:n].copy(), r2[:n].copy()
+
+
+def _pair_out(fname, a, b, cm, extra=2):
+    a_, ap = _u8(a)
+    b_, bp = _u8(b)
+    out = np.zeros(len(a_) + extra, np.uint8)
+    n = getattr(lib(), fname)(ap, bp, len(a_), C.byref(cm), out.ctypes.data_as(C.POINTER(C.c_uint8)))
+    if n < 0:
+        raise ValueError("median should not be 0")
+    return out[:n].copy()
+
+
+def ancestor_2(a, b, cm):
+    return _pair_out("orc_ancestor_2", a, b, cm)
+
+
+def median_2_no_gaps(a, b, cm):
+    return _pair_out("orc_median_2_no_gaps", a, b, cm)
+
+
+def median_2_with_gaps(a, b, cm):
+    return _pair_out("orc_median_2_with_gaps", a, b, cm)
+
+
+def worst_2(a, b, cm):
+    a_, ap = _u8(a)
+    b_, bp = _u8(b)
+    return lib().orc_worst_2(ap, bp, len(a_), C.byref(cm))
+
+
+def verify_2(a, b, cm):
+    a_, ap = _u8(a)
+    b_, bp = _u8(b)
+    return lib().orc_verify_2(ap, bp, len(a_), C.byref(cm))
+
+
+def align_affine_3(si, sj, cm, swaped):
+    """algn_CAML_align_affine_3 body: si must not be longer than sj.
+    Returns dict(cost, median, medianwg, resi, resj, stats)."""
+    a, ap = _u8(si)
+    b, bp = _u8(sj)
+    cap = len(a) + len(b) + 2
+    bufs = [np.zeros(cap, np.uint8) for _ in range(4)]
+    ml, mwl, al = C.c_int(), C.c_int(), C.c_int()
+    st = OrcStats()
+    P = lambda x: x.ctypes.data_as(C.POINTER(C.c_uint8))
+    c = lib().orc_align_affine_3(ap, len(a), bp, len(b), C.byref(cm), int(swaped),
+                                 P(bufs[0]), C.byref(ml), P(bufs[1]), C.byref(mwl),
+                                 P(bufs[2]), P(bufs[3]), C.byref(al), C.byref(st))
+    if c == -1 and len(b) < len(a):
+        raise ValueError("pass the shorter one as first")
+    return dict(cost=c, median=bufs[0][:ml.value].copy(), medianwg=bufs[1][:mwl.value].copy(),
+                resi=bufs[2][:al.value].copy(), resj=bufs[3][:al.value].copy(), stats=st)
+
+
+def cost_affine_3(si, sj, cm):
+    a, ap = _u8(si)
+    b, bp = _u8(sj)
+    return lib().orc_cost_affine_3(ap, len(a), bp, len(b), C.byref(cm))
+
+
+def count_gaps(s):
+    a, ap = _u8(s)
+    return lib().orc_count_gaps(ap, len(a))
+
+
+def deltawh(s1, s2, deltaw=None):
+    a, ap = _u8(s1)
+    b, bp = _u8(s2)
+    return lib().orc_deltawh(ap, len(a), bp, len(b), 0 if deltaw is None else 1,
+                             0 if deltaw is None else deltaw)
+
+
+def cost_2(s1, s2, cm, deltaw=None):
+    """Align.cost_2 (linear): returns (cost, dm, stats) for the shorter-first
+    orientation."""
+    a, ap = _u8(s1)
+    b, bp = _u8(s2)
+    n1, n2 = (len(a), len(b)) if len(a) <= len(b) else (len(b), len(a))
+    dm = np.zeros((n1, n2), np.uint16)
+    st = OrcStats()
+    c = lib().orc_cost_2(ap, len(a), bp, len(b), C.byref(cm), 0 if deltaw is None else 1,
+                         0 if deltaw is None else deltaw,
+                         dm.ctypes.data_as(C.POINTER(C.c_uint16)), C.byref(st))
+    return c, dm, st
+
+
+def align_2(s1, s2, cm):
+    """Align.align_2 ~first_gap:true, linear model: (s1p, s2p, cost, stats)."""
+    a, ap = _u8(s1)
+    b, bp = _u8(s2)
+    cap = len(a) + len(b)
+    r1 = np.zeros(cap, np.uint8)
+    r2 = np.zeros(cap, np.uint8)
+    n = C.c_int()
+    st = OrcStats()
+    c = lib().orc_align_2(ap, len(a), bp, len(b), C.byref(cm),
+                          r1.ctypes.data_as(C.POINTER(C.c_uint8)),
+                          r2.ctypes.data_as(C.POINTER(C.c_uint8)), C.byref(n), C.byref(st))
+    return r1[:n.value].copy(), r2[:n.value].copy(), c, st
+
+
+def nw_3d(s1, s2, s3, c3):
+    a, ap = _u8(s1)
+    b, bp = _u8(s2)
+    c, cp = _u8(s3)
+    dm = np.zeros(len(a) * len(b) * len(c), np.uint16)
+    cost = lib().orc_nw_3d(ap, len(a), bp, len(b), cp, len(c), C.byref(c3),
+                           dm.ctypes.data_as(C.POINTER(C.c_uint16)))
+    return cost, dm
+
+
+def backtrack_3d(s1, s2, s3, dm):
+    a, ap = _u8(s1)
+    b, bp = _u8(s2)
+    c, cp = _u8(s3)
+    cap = len(a) + len(b) + len(c)
+    outs = [np.zeros(cap, np.uint8) for _ in range(3)]
+    P = lambda x: x.ctypes.data_as(C.POINTER(C.c_uint8))
+    n = lib().orc_backtrack_3d(ap, len(a), bp, len(b), cp, len(c),
+                               np.ascontiguousarray(dm).ctypes.data_as(C.POINTER(C.c_uint16)),
+                               GAP, P(outs[0]), P(outs[1]), P(outs[2]))
+    if n < 0:
+        return None
+    return tuple(o[:n].copy() for o in outs)
+
+
+def median_3(a, b, c, c3):
+    a_, ap = _u8(a)
+    b_, bp = _u8(b)
+    c_, cp = _u8(c)
+    out = np.zeros(len(a_) + 1, np.uint8)
+    n = lib().orc_median_3(ap, bp, cp, len(a_), C.byref(c3), out.ctypes.data_as(C.POINTER(C.c_uint8)))
+    return out[:n].copy()

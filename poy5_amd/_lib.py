@@ -1,0 +1,112 @@
+"""ctypes binding of libpoyhip.so (the C ABI in include/poyhip.h).
+
+The library is built in-tree by ``poy5_amd/csrc/Makefile`` (see
+``__graft_entry__.build``).  Loading fails loudly when it is missing: there is
+no CPU fallback for any entry point.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpoyhip.so")
+
+POYHIP_OK = 0
+ERRORS = {
+    -1: "POYHIP_EINVAL", -2: "POYHIP_ENOMEM", -3: "POYHIP_EDEVICE",
+    -4: "POYHIP_ESTATE", -5: "POYHIP_ECAP", -6: "POYHIP_EBUSY",
+}
+
+OP_ALIGN_AFFINE = 1
+OP_COST_AFFINE = 2
+OP_ALIGN_2D = 3
+OP_COST_2D = 4
+
+
+class PoyhipCM(C.Structure):
+    _fields_ = [
+        ("lcm", C.c_int), ("gap", C.c_int), ("a_sz", C.c_int),
+        ("combinations", C.c_int), ("cost_model", C.c_int), ("gap_open", C.c_int),
+        ("cost", C.c_int * 2048), ("worst", C.c_int * 2048),
+        ("prepend", C.c_int * 64), ("tail", C.c_int * 64),
+        ("median", C.c_uint8 * 2048),
+    ]
+
+
+class PoyhipError(RuntimeError):
+    def __init__(self, code, msg=""):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+_lock = threading.Lock()
+_lib = None
+
+# name -> (restype, argtypes); every symbol here is declared in poyhip.h
+_u8p = C.POINTER(C.c_uint8)
+_ip = C.POINTER(C.c_int)
+_i64p = C.POINTER(C.c_int64)
+_vp = C.c_void_p
+SIGNATURES = {
+    "poyhip_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
+    "poyhip_destroy": (C.c_int, [_vp]),
+    "poyhip_last_error": (C.c_char_p, [_vp]),
+    "poyhip_set_cm": (C.c_int, [_vp, C.POINTER(PoyhipCM)]),
+    "poyhip_flush": (C.c_int, [_vp]),
+    "poyhip_cost_2": (C.c_int, [_vp, _u8p, C.c_int, _u8p, C.c_int, C.c_int, _ip]),
+    "poyhip_backtrace_2d": (C.c_int, [_vp, _u8p, C.c_int, _u8p, C.c_int, _u8p, _u8p,
+                                      C.c_int, _ip, C.c_int]),
+    "poyhip_align_2d": (C.c_int, [_vp, _u8p, C.c_int, _u8p, C.c_int, C.c_int, C.c_int,
+                                  _u8p, _u8p, C.c_int, _ip, _ip]),
+    "poyhip_cost_affine_3": (C.c_int, [_vp, _u8p, C.c_int, _u8p, C.c_int, _ip]),
+    "poyhip_align_affine_3": (C.c_int, [_vp, _u8p, C.c_int, _u8p, C.c_int, C.c_int,
+                                        _u8p, _ip, _u8p, _ip, _u8p, _u8p, _ip, C.c_int, _ip]),
+    "poyhip_ancestor_2": (C.c_int, [_vp, _u8p, _u8p, C.c_int, _u8p, C.c_int, _ip]),
+    "poyhip_median_2": (C.c_int, [_vp, _u8p, _u8p, C.c_int, _u8p, C.c_int, _ip]),
+    "poyhip_median_2_with_gaps": (C.c_int, [_vp, _u8p, _u8p, C.c_int, _u8p, C.c_int, _ip]),
+    "poyhip_worst_2": (C.c_int, [_vp, _u8p, _u8p, C.c_int, _ip]),
+    "poyhip_batch_create": (C.c_int, [_vp, C.c_int, C.c_int, _ip, _ip, _ip, C.c_size_t,
+                                      C.POINTER(_vp)]),
+    "poyhip_batch_destroy": (C.c_int, [_vp]),
+    "poyhip_batch_out_bytes": (C.c_size_t, [_vp]),
+    "poyhip_batch_out_offset": (C.c_int64, [_vp, C.c_int]),
+    "poyhip_batch_out_cap": (C.c_int, [_vp, C.c_int]),
+    "poyhip_batch_run": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_int]),
+    "poyhip_batch_stats": (C.c_int, [_vp, _i64p, _ip, _ip]),
+    "poyhip_batch_use_graph": (C.c_int, [_vp, C.c_int]),
+    "poyhip_batch_last_fill_ms": (C.c_float, [_vp]),
+    "poyhip_batch_last_total_ms": (C.c_float, [_vp]),
+    "poyhip_sync": (C.c_int, [_vp]),
+    "poyhip_stream": (_vp, [_vp]),
+    "poyhip_gen_pairs": (C.c_int, [C.c_int, C.c_int, C.c_uint64, _u8p, _i64p, _ip, _i64p]),
+}
+
+
+def lib():
+    """Loads libpoyhip.so; raises ImportError when it has not been built."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise ImportError(
+                    f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                    "(no CPU fallback exists)")
+            L = C.CDLL(LIB_PATH)
+            for name, (res, args) in SIGNATURES.items():
+                f = getattr(L, name)
+                f.restype = res
+                f.argtypes = args
+            _lib = L
+        return _lib
+
+
+def check(rc, ctx=None):
+    if rc != POYHIP_OK:
+        msg = ""
+        if ctx is not None:
+            m = lib().poyhip_last_error(ctx)
+            msg = m.decode() if m else ""
+        raise PoyhipError(rc, msg)
+    return rc

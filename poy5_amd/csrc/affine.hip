@@ -1,0 +1,573 @@
+// Banded affine-gap DO alignment (POY algn_CAML_align_affine_3,
+// src/algn.c:2359) on gfx950: one wavefront per pair, systolic 64-row
+// stripes, Ukkonen T-doubling inside the kernel, 7-bit directions packed in
+// one byte per cell, followed by a per-pair backtrace that reproduces
+// backtrace_aff (src/algn.c:1715) and its median lookups.
+#include "poyhip_device.h"
+#include "poyhip_layout.h"
+
+namespace poyhip {
+
+// Direction byte of one affine cell, decided at fill time for the pair's
+// swap flag:
+//   bits 0-1  move taken from the "todo" mode   (choose_dir on DO_* flags)
+//   bits 2-3  move after an align column        (choose_dir on ALIGN_TO_*)
+//   bit 4     END_VERTICAL | HORIZONTAL_EQ_VERTICAL  (vertical run stops)
+//   bit 5     END_HORIZONTAL | HORIZONTAL_EQ_VERTICAL (horizontal run stops)
+//   bit 6     END_BLOCK                          (gap-gap block stops)
+// Move codes: 0 vertical, 1 horizontal, 2 diagonal block, 3 align.
+
+struct AffFillArgs {
+  PairTable pt;
+  const CmDev* cm;
+  uint8_t* dm;
+  int* ws;            // one workspace slot of ws_stride ints per wavefront
+  long long ws_stride;
+  long long* stab;    // stripe tables
+  PairResult* res;
+  int* counter;
+};
+
+// Reads `p[j]` for j in [0, n); other j read p[0] (the value is unused).
+__device__ __forceinline__ int ldc(const uint8_t* p, int j, int n) {
+  return p[(unsigned)j < (unsigned)n ? j : 0];
+}
+
+template <bool kDm>
+__global__ __launch_bounds__(64) void k_affine_fill(AffFillArgs A) {
+  __shared__ int s_cost[1024];
+  __shared__ int s_prep[32];
+  const int lane = threadIdx.x;
+  for (int k = lane; k < 1024; k += 64) s_cost[k] = A.cm->cost[k];
+  if (lane < 32) s_prep[lane] = A.cm->prepend[lane];
+  __syncthreads();
+  const int go = A.cm->gap_open;
+  const int delta = A.cm->delta;
+
+  for (;;) {
+    int q = 0;
+    if (lane == 0) q = atomicAdd(A.counter, 1);
+    q = __builtin_amdgcn_readfirstlane(q);
+    if (q >= A.pt.n) break;
+    const int pid = A.pt.order[q];
+    const uint8_t* __restrict__ sa = A.pt.blob + rows_off(A.pt, pid);
+    const uint8_t* __restrict__ sb = A.pt.blob + cols_off(A.pt, pid);
+    const int La = A.pt.len_a[pid], Lb = A.pt.len_b[pid];
+    const int sw = A.pt.prio[pid];
+    const int leni = La - 1, lenj = Lb - 1, D = lenj - leni;
+    const int W1 = lenj + 1;
+    int* ws = A.ws + (long long)blockIdx.x * A.ws_stride;
+    uint8_t* dm = A.dm + (kDm ? A.pt.dm_off[pid] : 0);
+    int* EDA = ws;
+    int* bnd = ws + W1;
+    long long* stab = A.stab + A.pt.stab_off[pid];
+
+    if (leni == 0) {
+      // No DP rows: the cost is final_cost_matrix[lenj] as set by
+      // initialize_matrices_affine (src/algn.c:1866-1900).
+      int sum = 0;
+      for (int j = 1 + lane; j <= lenj; j += 64) sum += s_prep[sb[j]];
+      for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o);
+      int T = (D + 1) * delta, rounds = 0;
+      for (;;) {
+        const int p = (T - D) / 2;
+        rounds++;
+        const int newp = (2 * T - D) / 2;
+        if (65535 < p || newp - lenj + 1 >= 0) break;
+        T *= 2;
+      }
+      if (lane == 0) {
+        PairResult r;
+        r.cost = lenj == 0 ? 0 : go + sum;
+        r.gap_num = 65535;
+        r.newk = -1;
+        r.rounds = rounds;
+        r.full = 0;
+        r.pad = 0;
+        r.cells = 0;
+        A.res[pid] = r;
+      }
+      continue;
+    }
+
+    for (int j = lane; j <= lenj; j += 64) EDA[j] = kHigh;
+    int HA0 = go;
+    __threadfence_block();
+
+    int T = (D + 1) * delta;
+    int rounds = 0, newk = 0, cost = 0, gap_num = 0;
+    long long cells = 0;
+    for (;;) {
+      const int p = (T - D) / 2;
+      newk = p >= leni ? leni - 1 : p;
+      const int nstripes = (leni + 63) >> 6;
+      long long dmpos = 0;
+      int rc = 0, rgn = 0;
+      for (int b = 0; b < nstripes; b++) {
+        const int i0 = 1 + (b << 6);
+        const int ilast = imin(i0 + 63, leni);
+        const int i = i0 + lane;
+        const bool act = i <= leni;
+        const int jlo = imax(i0 - newk, 0);
+        const int jhi = (ilast + D + newk <= lenj - 1) ? ilast + D + newk : lenj;
+        const int nsteps = (jhi - jlo + 1) + (ilast - i0);
+        if (kDm && lane == 0) stab[b] = dmpos;
+
+        int ic = 0, ip = 0, si_ge = 0, si_go = 0, si_ng = 0, si_vext = 0;
+        int my_lo = 1 << 30, my_hi = -1, my_rb = -1;
+        if (act) {
+          ic = sa[i];
+          ip = sa[i - 1];
+          si_ge = s_cost[(ic << 5) + kGap];
+          si_go = has_gap_opening(i, ip, ic, go);
+          si_ng = ic & 15;
+          si_vext = (i > 1 && (ip & kGap) && !(ic & kGap)) ? si_go + si_ge : si_ge;
+          my_lo = imax(i - newk, 0);
+          my_rb = i + D + newk;
+          my_hi = (my_rb <= lenj - 1) ? my_rb : lenj;
+          cells += my_hi - my_lo + 1;
+        }
+        const int* rb = bnd + ((b & 1) ^ 1) * 5 * W1;
+        int* wb = bnd + (b & 1) * 5 * W1;
+        const bool is_bnd_row = (lane == 63);
+
+        // Up-left cell of the first step; lane 0 reads the row above the
+        // stripe, other lanes receive theirs through the pipeline.
+        int dV = kHigh, dCB = kHigh, dH = kHigh, dED = kHigh, dG = 0;
+        if (lane == 0 && jlo >= 1) {
+          if (b == 0) {
+            dV = kHigh; dCB = 0; dH = HA0; dED = EDA[0]; dG = 0;
+          } else {
+            dV = rb[jlo - 1]; dCB = rb[W1 + jlo - 1]; dH = rb[2 * W1 + jlo - 1];
+            dED = rb[3 * W1 + jlo - 1]; dG = rb[4 * W1 + jlo - 1];
+          }
+        }
+        int lH = kHigh, lCB = kHigh, lG = 0;
+        int oV = kHigh, oCB = kHigh, oH = kHigh, oED = kHigh, oG = 0;
+        int cb0run = 0;
+        int jprev = ldc(sb, jlo - lane - 1, Lb);
+        int jc_next = ldc(sb, jlo - lane, Lb);
+        // lane 0 prefetch of the row above for the first column
+        int nV = kHigh, nCB = kHigh, nH = kHigh, nED = kHigh, nG = 0;
+        if (lane == 0 && b > 0 && jlo <= lenj) {
+          nV = rb[jlo]; nCB = rb[W1 + jlo]; nH = rb[2 * W1 + jlo];
+          nED = rb[3 * W1 + jlo]; nG = rb[4 * W1 + jlo];
+        } else if (lane == 0 && b == 0 && jlo <= lenj) {
+          nED = EDA[jlo];
+        }
+        uint32_t pack = 0;
+        for (int s = 0; s < nsteps; s++) {
+          const int j = jlo + s - lane;
+          const int jc = jc_next;
+          jc_next = ldc(sb, j + 1, Lb);
+          // values of row i0-1 at column j for lane 0
+          int sV = nV, sCB = nCB, sH = nH, sED = nED, sG = nG;
+          if (lane == 0) {
+            if (b == 0) {
+              int cb0;
+              if (j == 0) {
+                cb0 = 0;
+              } else {
+                const int gr = s_prep[jc];
+                const int gop = has_gap_opening(j, jprev, jc, go);
+                cb0 = cb0run + (((jprev & kGap) && !(jc & kGap)) ? gop + gr : gr);
+              }
+              cb0run = cb0;
+              sV = kHigh;
+              sCB = cb0;
+              sH = (j == 0) ? HA0 : cb0;
+              sG = j & 0xFFFF;
+              if (j + 1 <= lenj) nED = EDA[j + 1];
+            } else if (j + 1 <= lenj) {
+              nV = rb[j + 1]; nCB = rb[W1 + j + 1]; nH = rb[2 * W1 + j + 1];
+              nED = rb[3 * W1 + j + 1]; nG = rb[4 * W1 + j + 1];
+            }
+          }
+          const int uV = wave_shr1(oV, sV);
+          const int uCB = wave_shr1(oCB, sCB);
+          const int uH = wave_shr1(oH, sH);
+          const int uED = wave_shr1(oED, sED);
+          const int uG = wave_shr1(oG, sG);
+          uint32_t code = 0;
+          if (act && j >= my_lo && j <= my_hi) {
+            const bool at_left = (j == my_lo), at_right = (j == my_rb);
+            const int jp_ed = at_left ? jc : jprev;
+            const int gr = s_prep[jc];
+            const int sj_go = has_gap_opening(j, jprev, jc, go);
+            const int shx = (j == 1) ? gr : (((jprev & kGap) && !(jc & kGap)) ? sj_go + gr : gr);
+            int H, V, E, C;
+            int endH = 0, endV = 0, endB = 0;
+            int a2v = 0, a2h = 0, a2d = 0;
+            if (!at_left) {
+              const int ext = lH + shx, opn = lCB + sj_go + gr;
+              if (ext < opn) H = ext; else { H = opn; endH = 1; }
+            } else {
+              H = kHigh;
+            }
+            if (!at_right) {
+              const int ext = uV + si_vext, opn = uCB + si_go + si_ge;
+              if (ext < opn) V = ext; else { V = opn; endV = 1; }
+            } else {
+              V = kHigh;
+            }
+            if (j > 0) {
+              const bool flag = (kGap & ic) && (kGap & jc);
+              const bool flag2 = !(kGap & ip) && !(kGap & jp_ed);
+              const int dg = flag ? 0 : kHigh;
+              const int od = flag ? (flag2 ? 0 : 2 * go) : kHigh;
+              const int ext = dED + dg, opn = dCB + od;
+              if (ext < opn) E = ext; else { E = opn; endB = 1; }
+              const int d = s_cost[(si_ng << 5) + (jc & 15)];
+              const int extra = (sj_go < si_go) ? si_go : sj_go;
+              const int fa = dCB + d;
+              const int fv = dV + d + ((ic == si_ng) ? 0 : sj_go);
+              const int fh = dH + d + ((jc == (jc & 15)) ? 0 : si_go);
+              const int fd = dED + d + extra;
+              int cb = fa, a2a = 1;
+              if (cb >= fv) { if (cb > fv) { cb = fv; a2a = 0; a2v = 1; } else a2v = 1; }
+              if (cb >= fh) { if (cb > fh) { cb = fh; a2a = 0; a2v = 0; a2h = 1; } else a2h = 1; }
+              if (cb >= fd) { if (cb > fd) { cb = fd; a2a = 0; a2v = 0; a2h = 0; a2d = 1; } else a2d = 1; }
+              (void)a2a;
+              C = cb;
+            } else {
+              C = kHigh;
+              E = kHigh;
+            }
+            // ASSIGN_MINIMUM (src/algn.c:1936-1981)
+            int f = H, dh = 1, dv = 0, dd = 0, da = 0;
+            if (f >= V) { if (f > V) { f = V; dh = 0; dv = 1; } else dv = 1; }
+            if (f >= E) { if (f > E) { f = E; dh = 0; dv = 0; dd = 1; } else dd = 1; }
+            if (f >= C) { if (f > C) { f = C; dh = 0; dv = 0; dd = 0; da = 1; } else da = 1; }
+            const int heqv = (f == H && H == V);
+            // algn_fill_gapnum (src/algn.c:127-176), u16 rows
+            const int pg1l = dG & 0xFFFF, pg2l = (int)((unsigned)dG >> 16);
+            const int pg1 = uG & 0xFFFF, pg2 = (int)((unsigned)uG >> 16);
+            const int g1l = lG & 0xFFFF, g2l = (int)((unsigned)lG >> 16);
+            const int al = dd | da;
+            int g1, g2;
+            if (al) {
+              if (dv && dh) { g1 = imax(imax(pg1l, g1l + 1), pg1); g2 = imax(imax(pg2l, g2l), pg2 + 1); }
+              else if (dv) { g1 = imax(pg1, pg1l); g2 = imax(pg2l, pg2 + 1); }
+              else if (dh) { g1 = imax(g1l + 1, pg1l); g2 = imax(pg2l, g2l); }
+              else { g1 = pg1l; g2 = pg2l; }
+            } else if (dh) {
+              if (dv) { g1 = imax(g1l + 1, pg1); g2 = imax(pg2 + 1, g2l); }
+              else { g1 = g1l + 1; g2 = g2l; }
+            } else {
+              g1 = pg1; g2 = pg2 + 1;
+            }
+            const int G = (g1 & 0xFFFF) | ((g2 & 0xFFFF) << 16);
+            const int todo = choose_dir(dd, dv, dh, sw);
+            const int amode = choose_dir(a2d, a2v, a2h, sw);
+            code = (uint32_t)(todo | (amode << 2) | ((endV | heqv) << 4) | ((endH | heqv) << 5) | (endB << 6));
+            oV = V; oCB = C; oH = H; oED = E; oG = G;
+            lH = H; lCB = C; lG = G;
+            if (i == leni && j == lenj) { rc = f; rgn = imax(g1 & 0xFFFF, g2 & 0xFFFF); }
+            // row-0 buffer persistence across rounds: the last even row of
+            // this round that covers column j leaves its ED value there.
+            if (!(i & 1) && i + 2 > imin(leni, j + newk)) EDA[j] = E;
+            if (is_bnd_row) {
+              wb[j] = V; wb[W1 + j] = C; wb[2 * W1 + j] = H; wb[3 * W1 + j] = E; wb[4 * W1 + j] = G;
+            }
+          }
+          if (kDm) {
+            pack |= code << ((s & 3) << 3);
+            if ((s & 3) == 3 || s == nsteps - 1) {
+              reinterpret_cast<uint32_t*>(dm + dmpos)[((s >> 2) << 6) + lane] = pack;
+              pack = 0;
+            }
+          }
+          dV = uV; dCB = uCB; dH = uH; dED = uED; dG = uG;
+          jprev = jc;
+        }
+        dmpos += (long long)((nsteps + 3) >> 2) * 256;
+        __threadfence_block();
+      }
+      const int src = (leni - 1) & 63;
+      cost = __shfl(rc, src);
+      gap_num = __shfl(rgn, src);
+      rounds++;
+      const int newp = (2 * T - D) / 2;
+      if (gap_num < p || newp - lenj + 1 >= 0) break;
+      if (imin(leni, newk) >= 2) HA0 = kHigh;
+      T *= 2;
+    }
+    for (int o = 32; o > 0; o >>= 1) cells += __shfl_xor(cells, o);
+    if (lane == 0) {
+      PairResult r;
+      r.cost = cost;
+      r.gap_num = gap_num;
+      r.newk = newk;
+      r.rounds = rounds;
+      r.full = 0;
+      r.pad = 0;
+      r.cells = cells;
+      A.res[pid] = r;
+    }
+  }
+}
+
+struct AffBtArgs {
+  PairTable pt;
+  const CmDev* cm;
+  const uint8_t* dm;
+  const long long* stab;
+  const PairResult* res;
+  uint8_t* out;
+  int* outlen;  // 4 per pair: aligned length, aligned length, median, median-with-gaps
+};
+
+// backtrace_aff (src/algn.c:1715-1819) with follow_*_affine (1531-1592) on
+// the packed direction bytes.  Outputs are written right to left into four
+// regions of capacity La+Lb+2 (resi, resj, median, medianwg).
+__global__ __launch_bounds__(64) void k_affine_backtrace(AffBtArgs A) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= A.pt.n) return;
+  const int pid = A.pt.order[q];
+  const uint8_t* sa = A.pt.blob + rows_off(A.pt, pid);
+  const uint8_t* sb = A.pt.blob + cols_off(A.pt, pid);
+  const int La = A.pt.len_a[pid], Lb = A.pt.len_b[pid];
+  const int geo = A.pt.swapped[pid];
+  const int cap = La + Lb + 2;
+  uint8_t* blk = A.out + A.pt.out_off[pid];
+  // regions in the caller's order: first sequence, second sequence
+  uint8_t* ri = geo ? blk + cap : blk;
+  uint8_t* rj = geo ? blk : blk + cap;
+  uint8_t* md = blk + 2 * cap;
+  uint8_t* mw = blk + 3 * cap;
+  int ni = 0, nj = 0, nm = 0, nw = 0;
+  const uint8_t* dm = A.dm + A.pt.dm_off[pid];
+  const long long* stab = A.stab + A.pt.stab_off[pid];
+  const int newk = A.res[pid].newk;
+  const long long dm_cap = aff_dm_bytes(La, Lb);
+  const uint8_t* med = A.cm->median;
+  int i = La - 1, j = Lb - 1;
+  int ic = sa[i], jc = sb[j];
+  int mode = 4;  // 0 vert, 1 horz, 2 diag, 3 align, 4 todo
+  int cb = -1, jlo = 0;
+  long long base = 0;
+#define PUT(buf, n, v) (buf)[cap - 1 - (n)++] = (uint8_t)(v)
+  while (i != 0 && j != 0) {
+    const int bb = (i - 1) >> 6;
+    if (bb != cb) {
+      cb = bb;
+      jlo = imax(1 + (bb << 6) - newk, 0);
+      base = stab[bb];
+    }
+    const int t = (i - 1) & 63;
+    const int s = j - jlo + t;
+    long long at = base + ((long long)(s >> 2) << 8) + (t << 2) + (s & 3);
+    // cells on the path are inside the band; the clamp only keeps a corrupt
+    // walk inside the pair's region
+    if (s < 0 || at >= dm_cap) at = 0;
+    const int code = dm[at];
+    if (mode == 4) {
+      mode = code & 3;
+    } else if (mode == 0) {
+      if (code & 16) mode = 4;
+      if (!(ic & kGap)) { PUT(md, nm, ic | kGap); PUT(mw, nw, ic | kGap); }
+      else PUT(mw, nw, kGap);
+      PUT(ri, ni, ic);
+      PUT(rj, nj, kGap);
+      i--;
+      ic = sa[i];
+    } else if (mode == 1) {
+      if (code & 32) mode = 4;
+      if (!(jc & kGap)) { PUT(md, nm, jc | kGap); PUT(mw, nw, jc | kGap); }
+      else PUT(mw, nw, kGap);
+      PUT(ri, ni, kGap);
+      PUT(rj, nj, jc);
+      j--;
+      jc = sb[j];
+    } else if (mode == 2) {
+      if (code & 64) mode = 4;
+      PUT(ri, ni, ic);
+      PUT(rj, nj, jc);
+      PUT(mw, nw, kGap);
+      i--; j--;
+      jc = sb[j];
+      ic = sa[i];
+    } else {
+      mode = (code >> 2) & 3;
+      const int prep = med[((ic & 15) << 5) + (jc & 15)];
+      PUT(md, nm, prep);
+      PUT(mw, nw, prep);
+      PUT(ri, ni, ic);
+      PUT(rj, nj, jc);
+      i--; j--;
+      jc = sb[j];
+      ic = sa[i];
+    }
+  }
+  while (i != 0) {
+    if (!(ic & kGap)) { PUT(md, nm, ic | kGap); PUT(mw, nw, ic | kGap); }
+    else PUT(mw, nw, kGap);
+    PUT(ri, ni, ic);
+    PUT(rj, nj, kGap);
+    i--;
+    ic = sa[i];
+  }
+  while (j != 0) {
+    if (!(jc & kGap)) { PUT(md, nm, jc | kGap); PUT(mw, nw, jc | kGap); }
+    else PUT(mw, nw, kGap);
+    PUT(ri, ni, kGap);
+    PUT(rj, nj, jc);
+    j--;
+    jc = sb[j];
+  }
+  PUT(ri, ni, kGap);
+  PUT(rj, nj, kGap);
+  PUT(mw, nw, kGap);
+  if (nm == 0 || md[cap - nm] != kGap) PUT(md, nm, kGap);
+#undef PUT
+  A.outlen[4 * pid + (geo ? 1 : 0)] = ni;
+  A.outlen[4 * pid + (geo ? 0 : 1)] = nj;
+  A.outlen[4 * pid + 2] = nm;
+  A.outlen[4 * pid + 3] = nw;
+}
+
+// algn_CAML_cost_affine_3 (src/algn.c:2457) -> algn_fill_plane_3_aff_nobt
+// (1987): full matrix, no directions, either order (rows = shorter).
+struct AffCostArgs {
+  PairTable pt;
+  const CmDev* cm;
+  int* ws;
+  long long ws_stride;
+  PairResult* res;
+  int* counter;
+};
+
+__global__ __launch_bounds__(64) void k_affine_cost_full(AffCostArgs A) {
+  __shared__ int s_cost[1024];
+  __shared__ int s_prep[32];
+  const int lane = threadIdx.x;
+  for (int k = lane; k < 1024; k += 64) s_cost[k] = A.cm->cost[k];
+  if (lane < 32) s_prep[lane] = A.cm->prepend[lane];
+  __syncthreads();
+  const int go = A.cm->gap_open;
+  for (;;) {
+    int q = 0;
+    if (lane == 0) q = atomicAdd(A.counter, 1);
+    q = __builtin_amdgcn_readfirstlane(q);
+    if (q >= A.pt.n) break;
+    const int pid = A.pt.order[q];
+    const uint8_t* __restrict__ sa = A.pt.blob + rows_off(A.pt, pid);
+    const uint8_t* __restrict__ sb = A.pt.blob + cols_off(A.pt, pid);
+    const int La = A.pt.len_a[pid], Lb = A.pt.len_b[pid];
+    const int leni = La - 1, lenj = Lb - 1;
+    const int W1 = lenj + 1;
+    int* bnd = A.ws + (long long)blockIdx.x * A.ws_stride;
+    int rc = 0;
+    if (leni == 0) {
+      // row 0 of initialize_matrices_affine_nobt: H = go + prefix sums
+      int sum = 0;
+      for (int j = 1 + lane; j <= lenj; j += 64) sum += s_prep[sb[j]];
+      for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o);
+      if (lane == 0) {
+        PairResult r{};
+        r.cost = lenj == 0 ? 0 : go + sum;  // min(H, V=go|HIGH, ED, CB) at lenj
+        if (lenj == 0) r.cost = 0;
+        r.newk = -1;
+        r.rounds = 1;
+        A.res[pid] = r;
+      }
+      continue;
+    }
+    const int nstripes = (leni + 63) >> 6;
+    for (int b = 0; b < nstripes; b++) {
+      const int i0 = 1 + (b << 6);
+      const int ilast = imin(i0 + 63, leni);
+      const int i = i0 + lane;
+      const bool act = i <= leni;
+      const int nsteps = W1 + (ilast - i0);
+      int ic = 0, ip = 0, si_ge = 0, si_go = 0, si_ng = 0, si_vext = 0;
+      if (act) {
+        ic = sa[i];
+        ip = sa[i - 1];
+        si_ge = s_cost[(ic << 5) + kGap];
+        si_go = has_gap_opening(i, ip, ic, go);
+        si_ng = ic & 15;
+        si_vext = (i > 1 && (ip & kGap) && !(ic & kGap)) ? si_go + si_ge : si_ge;
+      }
+      const int* rb = bnd + ((b & 1) ^ 1) * 4 * W1;
+      int* wb = bnd + (b & 1) * 4 * W1;
+      int dV = kHigh, dCB = kHigh, dH = kHigh, dED = kHigh;
+      int lH = kHigh, lCB = kHigh;
+      int oV = kHigh, oCB = kHigh, oH = kHigh, oED = kHigh;
+      int h0run = go;
+      int jprev = ldc(sb, -lane - 1, Lb);
+      int jc_next = ldc(sb, -lane, Lb);
+      for (int s = 0; s < nsteps; s++) {
+        const int j = s - lane;
+        const int jc = jc_next;
+        jc_next = ldc(sb, j + 1, Lb);
+        int sV = kHigh, sCB = kHigh, sH = kHigh, sED = kHigh;
+        if (lane == 0) {
+          if (b == 0) {
+            // row 0 (initialize_matrices_affine_nobt, src/algn.c:1822-1864)
+            if (j == 0) { sV = go; sCB = 0; sH = go; sED = kHigh; h0run = go; }
+            else { h0run += s_prep[jc]; sV = kHigh; sCB = kHigh; sH = h0run; sED = kHigh; }
+          } else if (j <= lenj) {
+            sV = rb[j]; sCB = rb[W1 + j]; sH = rb[2 * W1 + j]; sED = rb[3 * W1 + j];
+          }
+        }
+        const int uV = wave_shr1(oV, sV);
+        const int uCB = wave_shr1(oCB, sCB);
+        const int uH = wave_shr1(oH, sH);
+        const int uED = wave_shr1(oED, sED);
+        if (act && j >= 0 && j <= lenj) {
+          int H, V, E, C;
+          if (j == 0) {
+            const int r = uV + si_vext;
+            H = kHigh; E = kHigh; V = r; C = kHigh;
+          } else {
+            const int gr = s_prep[jc];
+            const int sj_go = has_gap_opening(j, jprev, jc, go);
+            const int shx = (j == 1) ? gr : (((jprev & kGap) && !(jc & kGap)) ? sj_go + gr : gr);
+            int ext = lH + shx, opn = lCB + sj_go + gr;
+            H = ext < opn ? ext : opn;
+            ext = uV + si_vext; opn = uCB + si_go + si_ge;
+            V = ext < opn ? ext : opn;
+            const bool flag = (kGap & ic) && (kGap & jc);
+            const bool flag2 = !(kGap & ip) && !(kGap & jprev);
+            const int dg = flag ? 0 : kHigh;
+            const int od = flag ? (flag2 ? 0 : 2 * go) : kHigh;
+            ext = dED + dg; opn = dCB + od;
+            E = ext < opn ? ext : opn;
+            const int d = s_cost[(si_ng << 5) + (jc & 15)];
+            const int extra = (sj_go < si_go) ? si_go : sj_go;
+            int cb = dCB + d;
+            const int fv = dV + d + ((ic == si_ng) ? 0 : sj_go);
+            const int fh = dH + d + ((jc == (jc & 15)) ? 0 : si_go);
+            const int fd = dED + d + extra;
+            if (cb > fv) cb = fv;
+            if (cb > fh) cb = fh;
+            if (cb > fd) cb = fd;
+            C = cb;
+          }
+          oV = V; oCB = C; oH = H; oED = E;
+          lH = H; lCB = C;
+          if (i == leni && j == lenj) rc = imin(imin(H, V), imin(E, C));
+          if (lane == 63) { wb[j] = V; wb[W1 + j] = C; wb[2 * W1 + j] = H; wb[3 * W1 + j] = E; }
+        }
+        dV = uV; dCB = uCB; dH = uH; dED = uED;
+        jprev = jc;
+      }
+      __threadfence_block();
+    }
+    const int cost = __shfl(rc, (leni - 1) & 63);
+    if (lane == 0) {
+      PairResult r{};
+      r.cost = cost;
+      r.newk = leni - 1;
+      r.rounds = 1;
+      r.cells = (long long)leni * (lenj + 1);
+      A.res[pid] = r;
+    }
+  }
+}
+
+template __global__ void k_affine_fill<true>(AffFillArgs);
+template __global__ void k_affine_fill<false>(AffFillArgs);
+
+}  // namespace poyhip

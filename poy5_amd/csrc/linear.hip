@@ -1,0 +1,336 @@
+// Non-affine DO alignment (POY algn_nw, src/algn.c:2972) on gfx950: full
+// plane or Ukkonen band with T-doubling (algn_fill_plane_2, 1134), one
+// wavefront per pair in systolic 64-row stripes.  Each cell keeps the move
+// backtrace_2d (3278) would take under the pair's swap flag (2 bits) plus
+// the number of deletions on the path backtrace_2d_gaps (978) would take, so
+// the band stop test needs no second backtrace.
+#include "poyhip_device.h"
+#include "poyhip_layout.h"
+
+namespace poyhip {
+
+struct LinFillArgs {
+  PairTable pt;
+  const CmDev* cm;
+  uint8_t* dm;
+  int* ws;            // one workspace slot of ws_stride ints per wavefront
+  long long ws_stride;
+  long long* stab;
+  PairResult* res;
+  int* counter;
+};
+
+__device__ __forceinline__ int ldc8(const uint8_t* p, int j, int n) {
+  return p[(unsigned)j < (unsigned)n ? j : 0];
+}
+
+// algn_fill_plane_2's choice between the full plane and the band
+// (src/algn.c:1134-1177) for width 50 and height 50 + (L1 - L2)
+// (algn_nw_limit, 2954).
+__device__ __forceinline__ bool lin_full_plane(int L1, int L2, int deltawh) {
+  int height = 50 + (L1 - L2) + deltawh;
+  if (height > L1) height = L1;
+  if ((float)L1 >= 1.5f * (float)L2) return true;
+  if (2 * height < L1) return false;
+  return 8 >= (L1 - height);
+}
+
+template <bool kDm>
+__global__ __launch_bounds__(64) void k_linear_fill(LinFillArgs A) {
+  __shared__ int s_cost[1024];
+  __shared__ int s_prep[32];
+  __shared__ int s_tail[32];
+  const int lane = threadIdx.x;
+  for (int k = lane; k < 1024; k += 64) s_cost[k] = A.cm->cost[k];
+  if (lane < 32) {
+    s_prep[lane] = A.cm->prepend[lane];
+    s_tail[lane] = A.cm->tail[lane];
+  }
+  __syncthreads();
+  const int delta = A.cm->delta;
+
+  for (;;) {
+    int q = 0;
+    if (lane == 0) q = atomicAdd(A.counter, 1);
+    q = __builtin_amdgcn_readfirstlane(q);
+    if (q >= A.pt.n) break;
+    const int pid = A.pt.order[q];
+    const uint8_t* __restrict__ sa = A.pt.blob + rows_off(A.pt, pid);
+    const uint8_t* __restrict__ sb = A.pt.blob + cols_off(A.pt, pid);
+    const int L1 = A.pt.len_a[pid], L2 = A.pt.len_b[pid];
+    const int sw = A.pt.prio[pid];
+    const bool full = lin_full_plane(L1, L2, A.pt.deltawh[pid]);
+    int* bnd = A.ws + (long long)blockIdx.x * A.ws_stride;
+    uint8_t* dm = A.dm + (kDm ? A.pt.dm_off[pid] : 0);
+    long long* stab = A.stab + A.pt.stab_off[pid];
+    const int Dd = L2 - L1;
+
+    int T = (Dd + 1) * delta;
+    int rounds = 0, newk = 0, cost = 0, gap_num = 0;
+    long long cells = 0;
+    for (;;) {
+      const int p = (T - Dd) / 2;
+      newk = full ? L1 - 1 : (p >= L1 ? L1 - 1 : p);
+      const int lfr = full ? L2 : imin(L2 - L1 + 1 + p, L2);
+      if (lane == 0) cells += lfr;
+      int rc = 0, rdc = 0;
+      if (L1 == 1) {
+        int sum = 0;
+        for (int j = 1 + lane; j < L2; j += 64) sum += s_prep[sb[j]];
+        for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o);
+        rc = sum;
+        rdc = 0;
+      }
+      const int nstripes = (L1 - 1 + 63) >> 6;
+      long long dmpos = 0;
+      for (int b = 0; b < nstripes; b++) {
+        const int i0 = 1 + (b << 6);
+        const int ilast = imin(i0 + 63, L1 - 1);
+        const int i = i0 + lane;
+        const bool act = i <= L1 - 1;
+        const int jlo = full ? 0 : imax(i0 - newk, 0);
+        const int jhi = full ? L2 - 1 : imin(ilast + Dd + newk, L2 - 1);
+        const int nsteps = (jhi - jlo + 1) + (ilast - i0);
+        if (kDm && lane == 0) stab[b] = dmpos;
+        int s1i = 0, c = 0, tl = 0, col0 = 0, lo = 1 << 30, hi = -1;
+        bool hasL = false, hasR = false;
+        if (act) {
+          s1i = sa[i];
+          c = s_cost[(s1i << 5) + kGap];
+          tl = s_tail[s1i];
+          col0 = full ? c : tl;
+          if (full) {
+            lo = 0; hi = L2 - 1;
+          } else {
+            hasL = (i - newk) > 0;
+            lo = hasL ? i - newk : 0;
+            hasR = (i + Dd + newk) <= L2 - 1;
+            hi = hasR ? i + Dd + newk : L2 - 1;
+          }
+          cells += hi - lo + 1;
+        }
+        const int* rb = bnd + ((b & 1) ^ 1) * 2 * L2;
+        int* wb = bnd + (b & 1) * 2 * L2;
+        int dmv = 0, ddc = 0;  // up-left
+        if (lane == 0 && jlo >= 1) {
+          if (b == 0) { dmv = 0; ddc = 0; }  // row 0, column jlo-1 == 0
+          else { dmv = rb[jlo - 1]; ddc = rb[L2 + jlo - 1]; }
+        }
+        int lmv = 0, ldc = 0;   // left
+        int omv = 0, odc = 0;   // handed to the next lane
+        int r0run = 0;          // row-0 prefix (stripe 0, lane 0)
+        int jc_next = ldc8(sb, jlo - lane, L2);
+        int nmv = 0, ndc = 0;
+        if (lane == 0 && b > 0 && jlo <= L2 - 1) { nmv = rb[jlo]; ndc = rb[L2 + jlo]; }
+        uint32_t pack = 0;
+        for (int s = 0; s < nsteps; s++) {
+          const int j = jlo + s - lane;
+          const int jc = jc_next;
+          jc_next = ldc8(sb, j + 1, L2);
+          int smv = nmv, sdc = ndc;
+          if (lane == 0) {
+            if (b == 0) {
+              if (j > 0) r0run += s_prep[jc];
+              smv = r0run;
+              sdc = 0;
+            } else if (j + 1 <= L2 - 1) {
+              nmv = rb[j + 1];
+              ndc = rb[L2 + j + 1];
+            }
+          }
+          const int umv = wave_shr1(omv, smv);
+          const int udc = wave_shr1(odc, sdc);
+          uint32_t mv = 0;
+          if (act && j >= lo && j <= hi) {
+            int val, fl;
+            if (j == 0) {
+              val = umv + col0;
+              fl = kDelete;
+            } else {
+              const int alg = s_cost[(s1i << 5) + jc];
+              const int gapj = s_cost[(kGap << 5) + jc];
+              if (hasL && j == lo) {
+                if (hasR && lo == hi) {
+                  val = dmv + alg;
+                  fl = kAlign;
+                } else {
+                  const int del = umv + c, aln = dmv + alg;
+                  val = imin(del, aln);
+                  fl = (del == val ? kDelete : 0) | (aln == val ? kAlign : 0);
+                }
+              } else if (hasR && j == hi) {
+                const int ins = lmv + gapj, aln = dmv + alg;
+                val = imin(ins, aln);
+                fl = (ins == val ? kInsert : 0) | (aln == val ? kAlign : 0);
+              } else {
+                const int del = umv + c, ins = lmv + gapj, aln = dmv + alg;
+                val = imin(imin(del, ins), aln);
+                fl = (aln == val ? kAlign : 0) | (ins == val ? kInsert : 0) | (del == val ? kDelete : 0);
+              }
+              if (!hasR && j == L2 - 1) {
+                // algn_fill_last_column (src/algn.c:654)
+                const int cst = tl + umv;
+                if (cst < val) { val = cst; fl = kDelete; }
+                else if (cst == val) fl |= kDelete;
+              }
+            }
+            // backtrace_2d_gaps order: align, insert, delete
+            const int dc = (fl & kAlign) ? ddc : ((fl & kInsert) ? ldc : udc + 1);
+            // backtrace_2d order: align, then delete/insert by swap flag
+            mv = (fl & kAlign) ? 0u
+                 : (sw ? ((fl & kInsert) ? 2u : 1u) : ((fl & kDelete) ? 1u : 2u));
+            omv = val; odc = dc;
+            lmv = val; ldc = dc;
+            if (i == L1 - 1 && j == L2 - 1) { rc = val; rdc = dc; }
+            if (lane == 63) { wb[j] = val; wb[L2 + j] = dc; }
+          }
+          if (kDm) {
+            pack |= mv << ((s & 15) << 1);
+            if ((s & 15) == 15 || s == nsteps - 1) {
+              reinterpret_cast<uint32_t*>(dm + dmpos)[((s >> 4) << 6) + lane] = pack;
+              pack = 0;
+            }
+          }
+          dmv = umv; ddc = udc;
+        }
+        dmpos += (long long)((nsteps + 15) >> 4) * 256;
+        __threadfence_block();
+      }
+      if (L1 > 1) {
+        const int src = (L1 - 2) & 63;
+        cost = __shfl(rc, src);
+        gap_num = __shfl(rdc, src) + Dd;
+      } else {
+        cost = rc;
+        gap_num = L2 - 1;
+      }
+      rounds++;
+      if (full) break;
+      const int newp = (2 * T - Dd) / 2;
+      if ((gap_num + 1) < p || newp - L2 + 1 >= 0) break;
+      T *= 2;
+    }
+    for (int o = 32; o > 0; o >>= 1) cells += __shfl_xor(cells, o);
+    if (lane == 0) {
+      PairResult r;
+      r.cost = cost;
+      r.gap_num = gap_num;
+      r.newk = newk;
+      r.rounds = rounds;
+      r.full = full ? 1 : 0;
+      r.pad = 0;
+      r.cells = cells;
+      A.res[pid] = r;
+    }
+  }
+}
+
+struct LinBtArgs {
+  PairTable pt;
+  const CmDev* cm;
+  const uint8_t* dm;
+  const long long* stab;
+  const PairResult* res;
+  uint8_t* out;
+  int* outlen;  // 4 per pair: r1, r2, ancestor, median with gaps
+};
+
+// backtrace_2d (src/algn.c:3278) over the packed moves, emitting the aligned
+// pair together with algn_ancestor_2 (3603) for the linear model and
+// seq_get_median_2d_with_gaps (src/seq.c:241).  r1 belongs to the DP rows'
+// sequence; medians are looked up in the caller's argument order.
+__global__ __launch_bounds__(64) void k_linear_backtrace(LinBtArgs A) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= A.pt.n) return;
+  const int pid = A.pt.order[q];
+  const uint8_t* sa = A.pt.blob + rows_off(A.pt, pid);
+  const uint8_t* sb = A.pt.blob + cols_off(A.pt, pid);
+  const int L1 = A.pt.len_a[pid], L2 = A.pt.len_b[pid];
+  const int sw = A.pt.swapped[pid];  // caller order of the median lookup
+  const int cap = (int)lin_out_cap(L1, L2);
+  uint8_t* blk = A.out + A.pt.out_off[pid];
+  uint8_t* r1 = sw ? blk + cap : blk;
+  uint8_t* r2 = sw ? blk : blk + cap;
+  uint8_t* an = blk + 2 * cap;
+  uint8_t* mw = blk + 3 * cap;
+  int n1 = 0, n2 = 0, na = 0, nw = 0;
+  const uint8_t* dm = A.dm + A.pt.dm_off[pid];
+  const long long* stab = A.stab + A.pt.stab_off[pid];
+  const PairResult R = A.res[pid];
+  const bool full = R.full != 0;
+  const int newk = R.newk;
+  const long long dm_cap = lin_dm_bytes(L1, L2);
+  const uint8_t* med = A.cm->median;
+  int i = L1 - 1, j = L2 - 1;
+  int cb = -1, jlo = 0;
+  long long base = 0;
+#define PUT(buf, n, v) (buf)[cap - 1 - (n)++] = (uint8_t)(v)
+  for (;;) {
+    int x, y, mv;
+    if (i == 0 && j == 0) {
+      mv = 0;
+    } else if (i == 0) {
+      mv = 2;
+    } else if (j == 0) {
+      mv = 1;
+    } else {
+      const int bb = (i - 1) >> 6;
+      if (bb != cb) {
+        cb = bb;
+        jlo = full ? 0 : imax(1 + (bb << 6) - newk, 0);
+        base = stab[bb];
+      }
+      const int t = (i - 1) & 63;
+      const int s = j - jlo + t;
+      long long at = base + ((((long long)(s >> 4) << 6) + t) << 2);
+      // path cells are inside the band; the clamp only keeps a corrupt walk
+      // inside the pair's region
+      if (s < 0 || at + 4 > dm_cap) at = 0;
+      const uint32_t w = *reinterpret_cast<const uint32_t*>(dm + at);
+      mv = (w >> ((s & 15) << 1)) & 3;
+    }
+    if (mv == 0) { x = sa[i]; y = sb[j]; }
+    else if (mv == 1) { x = sa[i]; y = kGap; }
+    else { x = kGap; y = sb[j]; }
+    PUT(r1, n1, x);
+    PUT(r2, n2, y);
+    const int m = sw ? med[(y << 5) + x] : med[(x << 5) + y];
+    if (m != kGap) PUT(an, na, m);
+    PUT(mw, nw, m);
+    if (i == 0 && j == 0) break;
+    if (mv == 0) { i--; j--; }
+    else if (mv == 1) i--;
+    else j--;
+  }
+  PUT(an, na, kGap);
+#undef PUT
+  A.outlen[4 * pid + (sw ? 1 : 0)] = n1;
+  A.outlen[4 * pid + (sw ? 0 : 1)] = n2;
+  A.outlen[4 * pid + 2] = na;
+  A.outlen[4 * pid + 3] = nw;
+}
+
+// Align.count_gaps (seq_CAML_count, src/seq.c:644) for both sequences and
+// Align.cost_2's deltawh (src/sequence.ml:868-936).  has_deltaw = 0 is the
+// ?deltaw=None call made by align_2; 1 is DOS.distance's ~deltaw.
+__global__ void k_linear_deltawh(PairTable pt, int* deltawh, int has_deltaw, const int* deltaw) {
+  const int pid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pid >= pt.npairs) return;
+  const uint8_t* sa = pt.blob + rows_off(pt, pid);
+  const uint8_t* sb = pt.blob + cols_off(pt, pid);
+  const int L1 = pt.len_a[pid], L2 = pt.len_b[pid];
+  int g1 = 0, g2 = 0;
+  for (int k = 0; k < L1; k++) g1 += (sa[k] >= kGap && (sa[k] & kGap)) ? 1 : 0;
+  for (int k = 0; k < L2; k++) g2 += (sb[k] >= kGap && (sb[k] & kGap)) ? 1 : 0;
+  const int dif = L1 - L2;
+  const int lower = L1 / 10;
+  int dw;
+  if (!has_deltaw) dw = dif < lower ? lower / 2 : 2;
+  else dw = dif < lower ? lower : deltaw[pid];
+  deltawh[pid] = imax(g1, g2) + dw;
+}
+
+template __global__ void k_linear_fill<true>(LinFillArgs);
+template __global__ void k_linear_fill<false>(LinFillArgs);
+
+}  // namespace poyhip

@@ -1,0 +1,950 @@
+// Host side of libpoyhip: the C ABI declared in include/poyhip.h.
+//
+// Single translation unit: the kernel sources are included here so every
+// launch sees its kernel's definition.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "poyhip.h"
+#include "poyhip_device.h"
+#include "poyhip_layout.h"
+
+#include "affine.hip"
+#include "linear.hip"
+#include "medians.hip"
+
+using namespace poyhip;
+
+namespace {
+
+__global__ void k_zero_ints(int* p, int n) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) p[k] = 0;
+}
+
+__global__ void k_extract_cost(const PairResult* res, int* cost, int n) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) cost[k] = res[k].cost;
+}
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+};
+
+}  // namespace
+
+struct poyhip_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  char err[512] = {0};
+  bool has_cm = false;
+  poyhip_cm cm;
+  CmDev* d_cm = nullptr;
+  int num_cus = 0;
+  // scratch of the per-pair drop-ins (grow-only, like Matrix.default)
+  DevBuf stage, dm, ws, stab, outblk;
+  std::vector<uint8_t> h_stage;
+  std::vector<uint8_t> h_out;
+  // state of the last poyhip_cost_2, read by poyhip_backtrace_2d
+  bool last2d = false;
+  int last_deltawh = 0;
+  std::vector<uint8_t> last_s1, last_s2;
+};
+
+namespace {
+
+int set_err(poyhip_ctx* c, int code, const char* fmt, ...) {
+  if (c) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(c->err, sizeof c->err, fmt, ap);
+    va_end(ap);
+  }
+  return code;
+}
+
+#define HIPCHK(ctx, x)                                                                 \
+  do {                                                                                 \
+    hipError_t e_ = (x);                                                               \
+    if (e_ != hipSuccess)                                                              \
+      return set_err((ctx), POYHIP_EDEVICE, "%s failed: %s", #x, hipGetErrorString(e_)); \
+  } while (0)
+
+int grow(poyhip_ctx* c, DevBuf& b, size_t n) {
+  if (n == 0) n = 64;
+  if (n <= b.cap) return POYHIP_OK;
+  if (b.p) (void)hipFree(b.p);
+  b.p = nullptr;
+  b.cap = 0;
+  if (hipMalloc(&b.p, n) != hipSuccess) {
+    b.p = nullptr;
+    return set_err(c, POYHIP_ENOMEM, "hipMalloc of %zu bytes failed", n);
+  }
+  b.cap = n;
+  return POYHIP_OK;
+}
+
+bool valid_seq(const uint8_t* s, int l) { return s != nullptr && l >= 1; }
+
+// Offsets of the one-pair descriptor block used by the drop-in calls.
+struct OneDesc {
+  int64_t off1, off2, dm_off, stab_off, out_off;
+  int len_a, len_b, swapped, prio, deltawh, order, counter, pad;
+};
+constexpr size_t kDescBytes = 128;
+static_assert(sizeof(OneDesc) <= kDescBytes, "descriptor block");
+
+PairTable one_table(void* dstage, const uint8_t* dblob) {
+  char* base = static_cast<char*>(dstage);
+  OneDesc* d = reinterpret_cast<OneDesc*>(base);
+  PairTable t;
+  t.blob = dblob;
+  t.off1 = &d->off1;
+  t.off2 = &d->off2;
+  t.len_a = &d->len_a;
+  t.len_b = &d->len_b;
+  t.swapped = &d->swapped;
+  t.prio = &d->prio;
+  t.deltawh = &d->deltawh;
+  t.dm_off = &d->dm_off;
+  t.stab_off = &d->stab_off;
+  t.out_off = &d->out_off;
+  t.order = &d->order;
+  t.n = 1;
+  t.npairs = 1;
+  return t;
+}
+
+// Copies the descriptor and the two sequences to the device in one transfer.
+// The rows sequence is `a` (la), the columns sequence `b` (lb).
+int stage_pair(poyhip_ctx* c, const uint8_t* a, int la, const uint8_t* b, int lb, int prio,
+               int deltawh, PairTable* t, int** counter) {
+  const size_t need = kDescBytes + (size_t)la + (size_t)lb;
+  int rc = grow(c, c->stage, need);
+  if (rc) return rc;
+  c->h_stage.resize(need);
+  OneDesc d;
+  memset(&d, 0, sizeof d);
+  d.off1 = 0;
+  d.off2 = la;
+  d.len_a = la;
+  d.len_b = lb;
+  d.swapped = 0;
+  d.prio = prio;
+  d.deltawh = deltawh;
+  memcpy(c->h_stage.data(), &d, sizeof d);
+  memcpy(c->h_stage.data() + kDescBytes, a, la);
+  memcpy(c->h_stage.data() + kDescBytes + la, b, lb);
+  HIPCHK(c, hipMemcpyAsync(c->stage.p, c->h_stage.data(), need, hipMemcpyHostToDevice, c->stream));
+  *t = one_table(c->stage.p, static_cast<const uint8_t*>(c->stage.p) + kDescBytes);
+  *counter = &reinterpret_cast<OneDesc*>(c->stage.p)->counter;
+  return POYHIP_OK;
+}
+
+// Result block of the drop-ins: PairResult, 4 lengths, then the output.
+constexpr size_t kResBytes = 64;
+
+int run_linear_one(poyhip_ctx* c, const uint8_t* s1, int l1, const uint8_t* s2, int l2,
+                   int deltawh, int prio, bool backtrace, PairResult* out_res,
+                   std::vector<uint8_t>* outblk, int outlen[4]) {
+  PairTable t;
+  int* counter;
+  int rc = stage_pair(c, s1, l1, s2, l2, prio, deltawh, &t, &counter);
+  if (rc) return rc;
+  const int64_t dmb = lin_dm_bytes(l1, l2);
+  const int64_t outb = lin_out_bytes(l1, l2);
+  if ((rc = grow(c, c->dm, dmb))) return rc;
+  if ((rc = grow(c, c->ws, lin_ws_ints(l2) * sizeof(int)))) return rc;
+  if ((rc = grow(c, c->stab, (n_stripes(l1) + 1) * sizeof(long long)))) return rc;
+  if ((rc = grow(c, c->outblk, kResBytes + outb))) return rc;
+  PairResult* dres = static_cast<PairResult*>(c->outblk.p);
+  int* doutlen = reinterpret_cast<int*>(static_cast<char*>(c->outblk.p) + sizeof(PairResult));
+  uint8_t* dout = static_cast<uint8_t*>(c->outblk.p) + kResBytes;
+  LinFillArgs A;
+  A.pt = t;
+  A.cm = c->d_cm;
+  A.dm = static_cast<uint8_t*>(c->dm.p);
+  A.ws = static_cast<int*>(c->ws.p);
+  A.ws_stride = 0;
+  A.stab = static_cast<long long*>(c->stab.p);
+  A.res = dres;
+  A.counter = counter;
+  hipLaunchKernelGGL(k_linear_fill<true>, dim3(1), dim3(64), 0, c->stream, A);
+  HIPCHK(c, hipGetLastError());
+  size_t back = sizeof(PairResult);
+  if (backtrace) {
+    LinBtArgs B;
+    B.pt = t;
+    B.cm = c->d_cm;
+    B.dm = A.dm;
+    B.stab = A.stab;
+    B.res = dres;
+    B.out = dout;
+    B.outlen = doutlen;
+    hipLaunchKernelGGL(k_linear_backtrace, dim3(1), dim3(64), 0, c->stream, B);
+    HIPCHK(c, hipGetLastError());
+    back = kResBytes + outb;
+  }
+  c->h_out.resize(back);
+  HIPCHK(c, hipMemcpyAsync(c->h_out.data(), c->outblk.p, back, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  memcpy(out_res, c->h_out.data(), sizeof(PairResult));
+  if (backtrace) {
+    memcpy(outlen, c->h_out.data() + sizeof(PairResult), 4 * sizeof(int));
+    outblk->assign(c->h_out.begin() + kResBytes, c->h_out.begin() + kResBytes + outb);
+  }
+  return POYHIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int poyhip_create(int device, poyhip_ctx** out) {
+  if (!out) return POYHIP_EINVAL;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev)
+    return POYHIP_EDEVICE;
+  poyhip_ctx* c = new poyhip_ctx();
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess) { delete c; return POYHIP_EDEVICE; }
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return POYHIP_EDEVICE;
+  }
+  if (hipMalloc(&c->d_cm, sizeof(CmDev)) != hipSuccess) {
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return POYHIP_ENOMEM;
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cus = prop.multiProcessorCount;
+  if (c->num_cus <= 0) c->num_cus = 256;
+  *out = c;
+  return POYHIP_OK;
+}
+
+int poyhip_flush(poyhip_ctx* c) {
+  if (!c) return POYHIP_EINVAL;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  for (DevBuf* b : {&c->stage, &c->dm, &c->ws, &c->stab, &c->outblk}) {
+    if (b->p) (void)hipFree(b->p);
+    b->p = nullptr;
+    b->cap = 0;
+  }
+  c->last2d = false;
+  return POYHIP_OK;
+}
+
+int poyhip_destroy(poyhip_ctx* c) {
+  if (!c) return POYHIP_EINVAL;
+  poyhip_flush(c);
+  if (c->d_cm) (void)hipFree(c->d_cm);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return POYHIP_OK;
+}
+
+const char* poyhip_last_error(const poyhip_ctx* c) { return c ? c->err : "null context"; }
+
+int poyhip_sync(poyhip_ctx* c) {
+  if (!c) return POYHIP_EINVAL;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return POYHIP_OK;
+}
+
+void* poyhip_stream(poyhip_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int poyhip_set_cm(poyhip_ctx* c, const poyhip_cm* cm) {
+  if (!c || !cm) return POYHIP_EINVAL;
+  if (cm->lcm != 5 || cm->gap != 16)
+    return set_err(c, POYHIP_EINVAL, "only 5-bit nucleotide tables (lcm 5, gap 16) are supported");
+  (void)hipSetDevice(c->device);
+  CmDev h;
+  memset(&h, 0, sizeof h);
+  memcpy(h.cost, cm->cost, sizeof h.cost);
+  memcpy(h.worst, cm->worst, sizeof h.worst);
+  memcpy(h.prepend, cm->prepend, sizeof h.prepend);
+  memcpy(h.tail, cm->tail, sizeof h.tail);
+  memcpy(h.median, cm->median, sizeof h.median);
+  h.gap_open = cm->gap_open;
+  // cm_get_min_non0_cost (src/cm.c:1064) scans all 2*32*32 entries
+  int m = 0x3fffffff;
+  for (int k = 0; k < 2048; k++)
+    if (cm->cost[k] > 0 && cm->cost[k] < m) m = cm->cost[k];
+  h.delta = m;
+  h.cost_model = cm->cost_model;
+  h.combinations = cm->combinations;
+  HIPCHK(c, hipMemcpyAsync(c->d_cm, &h, sizeof h, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->cm = *cm;
+  c->has_cm = true;
+  c->last2d = false;
+  return POYHIP_OK;
+}
+
+int poyhip_cost_2(poyhip_ctx* c, const uint8_t* s1, int l1, const uint8_t* s2, int l2,
+                  int deltawh, int* cost) {
+  if (!c || !cost || !valid_seq(s1, l1) || !valid_seq(s2, l2)) return POYHIP_EINVAL;
+  if (!c->has_cm) return set_err(c, POYHIP_ESTATE, "no cost matrix installed");
+  if (l1 > l2) return set_err(c, POYHIP_EINVAL, "pass the shorter sequence first");
+  (void)hipSetDevice(c->device);
+  PairResult r;
+  int outlen[4];
+  int rc = run_linear_one(c, s1, l1, s2, l2, deltawh, 0, false, &r, nullptr, outlen);
+  if (rc) return rc;
+  *cost = r.cost;
+  c->last2d = true;
+  c->last_deltawh = deltawh;
+  c->last_s1.assign(s1, s1 + l1);
+  c->last_s2.assign(s2, s2 + l2);
+  return POYHIP_OK;
+}
+
+int poyhip_backtrace_2d(poyhip_ctx* c, const uint8_t* s1, int l1, const uint8_t* s2, int l2,
+                        uint8_t* r1, uint8_t* r2, int cap, int* len, int swaped) {
+  if (!c || !r1 || !r2 || !len || !valid_seq(s1, l1) || !valid_seq(s2, l2)) return POYHIP_EINVAL;
+  if (!c->last2d || (int)c->last_s1.size() != l1 || (int)c->last_s2.size() != l2 ||
+      memcmp(c->last_s1.data(), s1, l1) || memcmp(c->last_s2.data(), s2, l2))
+    return set_err(c, POYHIP_ESTATE, "backtrace without a matching poyhip_cost_2");
+  if (cap < l1 + l2) return set_err(c, POYHIP_ECAP, "capacity %d < %d", cap, l1 + l2);
+  (void)hipSetDevice(c->device);
+  // The stored moves are resolved for one tie order; refill when the
+  // requested order differs (the fill is deterministic).
+  PairResult r;
+  int outlen[4];
+  std::vector<uint8_t> blk;
+  int rc = run_linear_one(c, s1, l1, s2, l2, c->last_deltawh, swaped ? 1 : 0, true, &r, &blk,
+                          outlen);
+  if (rc) return rc;
+  const int cp = (int)lin_out_cap(l1, l2);
+  *len = outlen[0];
+  memcpy(r1, blk.data() + cp - outlen[0], outlen[0]);
+  memcpy(r2, blk.data() + cp + cp - outlen[1], outlen[1]);
+  return POYHIP_OK;
+}
+
+int poyhip_align_2d(poyhip_ctx* c, const uint8_t* s1, int l1, const uint8_t* s2, int l2,
+                    int deltawh, int swaped, uint8_t* r1, uint8_t* r2, int cap, int* len,
+                    int* cost) {
+  if (!c || !cost || !r1 || !r2 || !len || !valid_seq(s1, l1) || !valid_seq(s2, l2))
+    return POYHIP_EINVAL;
+  if (!c->has_cm) return set_err(c, POYHIP_ESTATE, "no cost matrix installed");
+  if (l1 > l2) return set_err(c, POYHIP_EINVAL, "pass the shorter sequence first");
+  if (cap < l1 + l2) return set_err(c, POYHIP_ECAP, "capacity %d < %d", cap, l1 + l2);
+  (void)hipSetDevice(c->device);
+  PairResult r;
+  int outlen[4];
+  std::vector<uint8_t> blk;
+  int rc = run_linear_one(c, s1, l1, s2, l2, deltawh, swaped ? 1 : 0, true, &r, &blk, outlen);
+  if (rc) return rc;
+  const int cp = (int)lin_out_cap(l1, l2);
+  *cost = r.cost;
+  *len = outlen[0];
+  memcpy(r1, blk.data() + cp - outlen[0], outlen[0]);
+  memcpy(r2, blk.data() + cp + cp - outlen[1], outlen[1]);
+  c->last2d = true;
+  c->last_deltawh = deltawh;
+  c->last_s1.assign(s1, s1 + l1);
+  c->last_s2.assign(s2, s2 + l2);
+  return POYHIP_OK;
+}
+
+int poyhip_cost_affine_3(poyhip_ctx* c, const uint8_t* si, int li, const uint8_t* sj, int lj,
+                         int* cost) {
+  if (!c || !cost || !valid_seq(si, li) || !valid_seq(sj, lj)) return POYHIP_EINVAL;
+  if (!c->has_cm) return set_err(c, POYHIP_ESTATE, "no cost matrix installed");
+  (void)hipSetDevice(c->device);
+  const uint8_t *a = si, *b = sj;
+  int la = li, lb = lj;
+  if (li > lj) { a = sj; b = si; la = lj; lb = li; }
+  PairTable t;
+  int* counter;
+  int rc = stage_pair(c, a, la, b, lb, 0, 0, &t, &counter);
+  if (rc) return rc;
+  if ((rc = grow(c, c->ws, affcost_ws_ints(lb) * sizeof(int)))) return rc;
+  if ((rc = grow(c, c->outblk, kResBytes))) return rc;
+  AffCostArgs A;
+  A.pt = t;
+  A.cm = c->d_cm;
+  A.ws = static_cast<int*>(c->ws.p);
+  A.ws_stride = 0;
+  A.res = static_cast<PairResult*>(c->outblk.p);
+  A.counter = counter;
+  hipLaunchKernelGGL(k_affine_cost_full, dim3(1), dim3(64), 0, c->stream, A);
+  HIPCHK(c, hipGetLastError());
+  PairResult r;
+  HIPCHK(c, hipMemcpyAsync(&r, c->outblk.p, sizeof r, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  *cost = r.cost;
+  return POYHIP_OK;
+}
+
+int poyhip_align_affine_3(poyhip_ctx* c, const uint8_t* si, int li, const uint8_t* sj, int lj,
+                          int swaped, uint8_t* median, int* median_len, uint8_t* medianwg,
+                          int* medianwg_len, uint8_t* resi, uint8_t* resj, int* res_len, int cap,
+                          int* cost) {
+  if (!c || !cost || !median || !median_len || !medianwg || !medianwg_len || !resi || !resj ||
+      !res_len || !valid_seq(si, li) || !valid_seq(sj, lj))
+    return POYHIP_EINVAL;
+  if (!c->has_cm) return set_err(c, POYHIP_ESTATE, "no cost matrix installed");
+  if (lj < li) return set_err(c, POYHIP_EINVAL, "pass the shorter one as first");
+  if (cap < li + lj + 2) return set_err(c, POYHIP_ECAP, "capacity %d < %d", cap, li + lj + 2);
+  (void)hipSetDevice(c->device);
+  PairTable t;
+  int* counter;
+  int rc = stage_pair(c, si, li, sj, lj, swaped ? 1 : 0, 0, &t, &counter);
+  if (rc) return rc;
+  const int64_t dmb = aff_dm_bytes(li, lj);
+  const int64_t outb = aff_out_bytes(li, lj);
+  if ((rc = grow(c, c->dm, dmb))) return rc;
+  if ((rc = grow(c, c->ws, aff_ws_ints(lj) * sizeof(int)))) return rc;
+  if ((rc = grow(c, c->stab, (n_stripes(li) + 1) * sizeof(long long)))) return rc;
+  if ((rc = grow(c, c->outblk, kResBytes + outb))) return rc;
+  PairResult* dres = static_cast<PairResult*>(c->outblk.p);
+  int* doutlen = reinterpret_cast<int*>(static_cast<char*>(c->outblk.p) + sizeof(PairResult));
+  uint8_t* dout = static_cast<uint8_t*>(c->outblk.p) + kResBytes;
+  AffFillArgs A;
+  A.pt = t;
+  A.cm = c->d_cm;
+  A.dm = static_cast<uint8_t*>(c->dm.p);
+  A.ws = static_cast<int*>(c->ws.p);
+  A.ws_stride = 0;
+  A.stab = static_cast<long long*>(c->stab.p);
+  A.res = dres;
+  A.counter = counter;
+  hipLaunchKernelGGL(k_affine_fill<true>, dim3(1), dim3(64), 0, c->stream, A);
+  HIPCHK(c, hipGetLastError());
+  AffBtArgs B;
+  B.pt = t;
+  B.cm = c->d_cm;
+  B.dm = A.dm;
+  B.stab = A.stab;
+  B.res = dres;
+  B.out = dout;
+  B.outlen = doutlen;
+  hipLaunchKernelGGL(k_affine_backtrace, dim3(1), dim3(64), 0, c->stream, B);
+  HIPCHK(c, hipGetLastError());
+  const size_t back = kResBytes + outb;
+  c->h_out.resize(back);
+  HIPCHK(c, hipMemcpyAsync(c->h_out.data(), c->outblk.p, back, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  PairResult r;
+  int ol[4];
+  memcpy(&r, c->h_out.data(), sizeof r);
+  memcpy(ol, c->h_out.data() + sizeof(PairResult), sizeof ol);
+  const uint8_t* blk = c->h_out.data() + kResBytes;
+  const int cp = li + lj + 2;
+  memcpy(resi, blk + cp - ol[0], ol[0]);
+  memcpy(resj, blk + 2 * cp - ol[1], ol[1]);
+  memcpy(median, blk + 3 * cp - ol[2], ol[2]);
+  memcpy(medianwg, blk + 4 * cp - ol[3], ol[3]);
+  *res_len = ol[0];
+  *median_len = ol[2];
+  *medianwg_len = ol[3];
+  *cost = r.cost;
+  return POYHIP_OK;
+}
+
+static int pair_median_op(poyhip_ctx* c, const uint8_t* a, const uint8_t* b, int len, int op,
+                          uint8_t* out, int cap, int* out_len, int* value) {
+  if (!c || !a || !b || len < 0) return POYHIP_EINVAL;
+  if (!c->has_cm) return set_err(c, POYHIP_ESTATE, "no cost matrix installed");
+  (void)hipSetDevice(c->device);
+  const size_t need = kDescBytes + 2 * (size_t)len + 8;
+  int rc = grow(c, c->stage, need);
+  if (rc) return rc;
+  const int ocap = len + 2;
+  if ((rc = grow(c, c->outblk, kResBytes + ocap))) return rc;
+  c->h_stage.resize(need);
+  memcpy(c->h_stage.data() + kDescBytes, a, len);
+  memcpy(c->h_stage.data() + kDescBytes + len, b, len);
+  HIPCHK(c, hipMemcpyAsync(c->stage.p, c->h_stage.data(), need, hipMemcpyHostToDevice, c->stream));
+  const uint8_t* da = static_cast<const uint8_t*>(c->stage.p) + kDescBytes;
+  int* dres = static_cast<int*>(c->outblk.p);
+  uint8_t* dout = static_cast<uint8_t*>(c->outblk.p) + kResBytes;
+  hipLaunchKernelGGL(k_pair_median, dim3(1), dim3(64), 0, c->stream, c->d_cm, da, da + len, len,
+                     op, dout, ocap, dres);
+  HIPCHK(c, hipGetLastError());
+  c->h_out.resize(kResBytes + ocap);
+  HIPCHK(c, hipMemcpyAsync(c->h_out.data(), c->outblk.p, kResBytes + ocap, hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  int res[2];
+  memcpy(res, c->h_out.data(), sizeof res);
+  if (res[1]) return set_err(c, POYHIP_EINVAL, "median should not be 0");
+  if (op == kWorst) {
+    *value = res[0];
+    return POYHIP_OK;
+  }
+  if (res[0] > cap) return set_err(c, POYHIP_ECAP, "capacity %d < %d", cap, res[0]);
+  memcpy(out, c->h_out.data() + kResBytes + ocap - res[0], res[0]);
+  *out_len = res[0];
+  return POYHIP_OK;
+}
+
+int poyhip_ancestor_2(poyhip_ctx* c, const uint8_t* a, const uint8_t* b, int len, uint8_t* out,
+                      int cap, int* out_len) {
+  if (!out || !out_len) return POYHIP_EINVAL;
+  return pair_median_op(c, a, b, len, kAncestor, out, cap, out_len, nullptr);
+}
+
+int poyhip_median_2(poyhip_ctx* c, const uint8_t* a, const uint8_t* b, int len, uint8_t* out,
+                    int cap, int* out_len) {
+  if (!out || !out_len) return POYHIP_EINVAL;
+  return pair_median_op(c, a, b, len, kMedianNoGaps, out, cap, out_len, nullptr);
+}
+
+int poyhip_median_2_with_gaps(poyhip_ctx* c, const uint8_t* a, const uint8_t* b, int len,
+                              uint8_t* out, int cap, int* out_len) {
+  if (!out || !out_len) return POYHIP_EINVAL;
+  return pair_median_op(c, a, b, len, kMedianWithGaps, out, cap, out_len, nullptr);
+}
+
+int poyhip_worst_2(poyhip_ctx* c, const uint8_t* a, const uint8_t* b, int len, int* cost) {
+  if (!cost) return POYHIP_EINVAL;
+  return pair_median_op(c, a, b, len, kWorst, nullptr, 0, nullptr, cost);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batches
+// ---------------------------------------------------------------------------
+
+struct poyhip_batch {
+  poyhip_ctx* ctx = nullptr;
+  int op = 0;
+  int n = 0;
+  std::vector<int> la, lb, swapped;
+  std::vector<int64_t> out_off, dm_off, stab_off;
+  std::vector<int> order;
+  struct Chunk {
+    int begin, count;
+    int64_t dm_bytes;
+  };
+  std::vector<Chunk> chunks;
+  int has_deltaw = 0;
+  // device
+  void* d_tab = nullptr;  // all descriptor arrays in one allocation
+  int64_t *d_dm_off = nullptr, *d_stab_off = nullptr, *d_out_off = nullptr;
+  int *d_la = nullptr, *d_lb = nullptr, *d_sw = nullptr, *d_dw = nullptr, *d_dwin = nullptr;
+  int* d_order = nullptr;
+  int* d_counters = nullptr;
+  PairResult* d_res = nullptr;
+  uint8_t* d_dm = nullptr;
+  int* d_ws = nullptr;
+  long long* d_stab = nullptr;
+  int64_t dm_cap = 0;
+  int grid = 0;
+  long long ws_stride = 0;
+  size_t out_bytes = 0;
+  // timing
+  std::vector<hipEvent_t> ev;
+  float fill_ms = 0.f, total_ms = 0.f;
+  // graph replay
+  bool use_graph = false;
+  hipGraphExec_t gexec = nullptr;
+  const void* g_key[6] = {nullptr};
+};
+
+namespace {
+
+bool op_is_affine(int op) { return op == POYHIP_OP_ALIGN_AFFINE || op == POYHIP_OP_COST_AFFINE; }
+bool op_has_dm(int op) { return op == POYHIP_OP_ALIGN_AFFINE || op == POYHIP_OP_ALIGN_2D; }
+
+int64_t pair_dm_bytes(int op, int la, int lb) {
+  if (op == POYHIP_OP_ALIGN_AFFINE) return aff_dm_bytes(la, lb);
+  if (op == POYHIP_OP_ALIGN_2D) return lin_dm_bytes(la, lb);
+  return 0;
+}
+int64_t pair_out_bytes(int op, int la, int lb) {
+  if (op == POYHIP_OP_ALIGN_AFFINE) return aff_out_bytes(la, lb);
+  if (op == POYHIP_OP_ALIGN_2D) return lin_out_bytes(la, lb);
+  return 0;
+}
+int64_t ws_ints(int op, int lb) {
+  if (op == POYHIP_OP_ALIGN_AFFINE) return aff_ws_ints(lb);
+  if (op == POYHIP_OP_COST_AFFINE) return affcost_ws_ints(lb);
+  return lin_ws_ints(lb);
+}
+
+void batch_free_device(poyhip_batch* b) {
+  for (void* p : {(void*)b->d_tab, (void*)b->d_res, (void*)b->d_dm, (void*)b->d_ws,
+                  (void*)b->d_stab, (void*)b->d_counters})
+    if (p) (void)hipFree(p);
+  b->d_tab = nullptr;
+  b->d_res = nullptr;
+  b->d_dm = nullptr;
+  b->d_ws = nullptr;
+  b->d_stab = nullptr;
+  b->d_counters = nullptr;
+  for (hipEvent_t e : b->ev) (void)hipEventDestroy(e);
+  b->ev.clear();
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  b->gexec = nullptr;
+}
+
+PairTable batch_table(const poyhip_batch* b, const uint8_t* blob, const int64_t* off1,
+                      const int64_t* off2, int chunk) {
+  PairTable t;
+  t.blob = blob;
+  t.off1 = off1;
+  t.off2 = off2;
+  t.len_a = b->d_la;
+  t.len_b = b->d_lb;
+  t.swapped = b->d_sw;
+  t.prio = b->d_sw;
+  t.deltawh = b->d_dw;
+  t.dm_off = b->d_dm_off;
+  t.stab_off = b->d_stab_off;
+  t.out_off = b->d_out_off;
+  const auto& ch = b->chunks[chunk < 0 ? 0 : chunk];
+  t.order = b->d_order + (chunk < 0 ? 0 : ch.begin);
+  t.n = chunk < 0 ? b->n : ch.count;
+  t.npairs = b->n;
+  return t;
+}
+
+// Enqueues every launch of one run on the context stream.
+int batch_enqueue(poyhip_batch* b, const uint8_t* blob, const int64_t* off1, const int64_t* off2,
+                  int* d_cost, uint8_t* d_out, int* d_outlen, bool timing) {
+  poyhip_ctx* c = b->ctx;
+  hipStream_t st = c->stream;
+  const int nch = (int)b->chunks.size();
+  if (timing) HIPCHK(c, hipEventRecord(b->ev[2 * nch], st));
+  hipLaunchKernelGGL(k_zero_ints, dim3((nch + 63) / 64), dim3(64), 0, st, b->d_counters, nch);
+  if (!op_is_affine(b->op)) {
+    PairTable all = batch_table(b, blob, off1, off2, -1);
+    hipLaunchKernelGGL(k_linear_deltawh, dim3((b->n + 255) / 256), dim3(256), 0, st, all, b->d_dw,
+                       b->has_deltaw, b->d_dwin);
+  }
+  for (int k = 0; k < nch; k++) {
+    PairTable t = batch_table(b, blob, off1, off2, k);
+    if (timing) HIPCHK(c, hipEventRecord(b->ev[2 * k], st));
+    switch (b->op) {
+      case POYHIP_OP_ALIGN_AFFINE:
+      case POYHIP_OP_COST_AFFINE: {
+        if (b->op == POYHIP_OP_COST_AFFINE) {
+          AffCostArgs A;
+          A.pt = t;
+          A.cm = c->d_cm;
+          A.ws = b->d_ws;
+          A.ws_stride = b->ws_stride;
+          A.res = b->d_res;
+          A.counter = b->d_counters + k;
+          hipLaunchKernelGGL(k_affine_cost_full, dim3(b->grid), dim3(64), 0, st, A);
+        } else {
+          AffFillArgs A;
+          A.pt = t;
+          A.cm = c->d_cm;
+          A.dm = b->d_dm;
+          A.ws = b->d_ws;
+          A.ws_stride = b->ws_stride;
+          A.stab = b->d_stab;
+          A.res = b->d_res;
+          A.counter = b->d_counters + k;
+          hipLaunchKernelGGL(k_affine_fill<true>, dim3(b->grid), dim3(64), 0, st, A);
+        }
+        break;
+      }
+      default: {
+        LinFillArgs A;
+        A.pt = t;
+        A.cm = c->d_cm;
+        A.dm = b->d_dm;
+        A.ws = b->d_ws;
+        A.ws_stride = b->ws_stride;
+        A.stab = b->d_stab;
+        A.res = b->d_res;
+        A.counter = b->d_counters + k;
+        if (b->op == POYHIP_OP_ALIGN_2D)
+          hipLaunchKernelGGL(k_linear_fill<true>, dim3(b->grid), dim3(64), 0, st, A);
+        else
+          hipLaunchKernelGGL(k_linear_fill<false>, dim3(b->grid), dim3(64), 0, st, A);
+      }
+    }
+    HIPCHK(c, hipGetLastError());
+    if (timing) HIPCHK(c, hipEventRecord(b->ev[2 * k + 1], st));
+    if (b->op == POYHIP_OP_ALIGN_AFFINE) {
+      AffBtArgs B;
+      B.pt = t;
+      B.cm = c->d_cm;
+      B.dm = b->d_dm;
+      B.stab = b->d_stab;
+      B.res = b->d_res;
+      B.out = d_out;
+      B.outlen = d_outlen;
+      hipLaunchKernelGGL(k_affine_backtrace, dim3((t.n + 63) / 64), dim3(64), 0, st, B);
+    } else if (b->op == POYHIP_OP_ALIGN_2D) {
+      LinBtArgs B;
+      B.pt = t;
+      B.cm = c->d_cm;
+      B.dm = b->d_dm;
+      B.stab = b->d_stab;
+      B.res = b->d_res;
+      B.out = d_out;
+      B.outlen = d_outlen;
+      hipLaunchKernelGGL(k_linear_backtrace, dim3((t.n + 63) / 64), dim3(64), 0, st, B);
+    }
+    HIPCHK(c, hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_extract_cost, dim3((b->n + 255) / 256), dim3(256), 0, st, b->d_res, d_cost,
+                     b->n);
+  HIPCHK(c, hipGetLastError());
+  if (timing) HIPCHK(c, hipEventRecord(b->ev[2 * nch + 1], st));
+  return POYHIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int poyhip_batch_create(poyhip_ctx* c, int op, int n, const int* len1, const int* len2,
+                        const int* deltaw, size_t dm_budget, poyhip_batch** out) {
+  if (!c || !out || n <= 0 || !len1 || !len2) return POYHIP_EINVAL;
+  if (op < POYHIP_OP_ALIGN_AFFINE || op > POYHIP_OP_COST_2D)
+    return set_err(c, POYHIP_EINVAL, "unknown batch op %d", op);
+  if (!c->has_cm) return set_err(c, POYHIP_ESTATE, "no cost matrix installed");
+  (void)hipSetDevice(c->device);
+  *out = nullptr;
+  poyhip_batch* b = new poyhip_batch();
+  b->ctx = c;
+  b->op = op;
+  b->n = n;
+  b->la.resize(n);
+  b->lb.resize(n);
+  b->swapped.resize(n);
+  b->out_off.resize(n);
+  b->dm_off.resize(n);
+  b->stab_off.resize(n);
+  int maxlb = 1;
+  std::vector<double> work(n);
+  for (int k = 0; k < n; k++) {
+    if (len1[k] < 1 || len2[k] < 1) {
+      delete b;
+      return set_err(c, POYHIP_EINVAL, "pair %d has an empty sequence", k);
+    }
+    const bool sw = len1[k] > len2[k];
+    b->swapped[k] = sw ? 1 : 0;
+    b->la[k] = sw ? len2[k] : len1[k];
+    b->lb[k] = sw ? len1[k] : len2[k];
+    maxlb = std::max(maxlb, b->lb[k]);
+    work[k] = (double)b->la[k] * (double)b->lb[k];
+  }
+  // largest first (LPT order for the persistent kernels)
+  b->order.resize(n);
+  std::iota(b->order.begin(), b->order.end(), 0);
+  std::stable_sort(b->order.begin(), b->order.end(),
+                   [&](int x, int y) { return work[x] > work[y]; });
+  // output block
+  int64_t o = 0;
+  for (int k = 0; k < n; k++) {
+    b->out_off[k] = o;
+    o += (pair_out_bytes(op, b->la[k], b->lb[k]) + 15) & ~(int64_t)15;
+  }
+  b->out_bytes = (size_t)o;
+  // stripe tables
+  int64_t so = 0;
+  for (int k = 0; k < n; k++) {
+    b->stab_off[k] = so;
+    so += n_stripes(b->la[k]) + 1;
+  }
+  // chunks by direction-store budget
+  if (dm_budget == 0) {
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) fr = (size_t)8 << 30;
+    dm_budget = std::min(fr / 2, (size_t)96 << 30);
+  }
+  int64_t cur = 0;
+  int begin = 0;
+  for (int q = 0; q < n; q++) {
+    const int k = b->order[q];
+    const int64_t sz = (pair_dm_bytes(op, b->la[k], b->lb[k]) + 255) & ~(int64_t)255;
+    if (cur > 0 && cur + sz > (int64_t)dm_budget) {
+      b->chunks.push_back({begin, q - begin, cur});
+      begin = q;
+      cur = 0;
+    }
+    b->dm_off[k] = cur;
+    cur += sz;
+  }
+  b->chunks.push_back({begin, n - begin, cur});
+  for (const auto& ch : b->chunks) b->dm_cap = std::max(b->dm_cap, ch.dm_bytes);
+  // persistent grid: waves resident across the device
+  b->grid = std::min(n, c->num_cus * 16);
+  if (b->grid < 1) b->grid = 1;
+  b->ws_stride = (ws_ints(op, maxlb) + 63) & ~63LL;
+  // device allocations
+  const size_t tab_bytes = (size_t)n * (3 * 8 + 6 * 4) + 1024;
+  int rc = POYHIP_OK;
+  auto fail = [&](int code, const char* what) {
+    batch_free_device(b);
+    delete b;
+    return set_err(c, code, "batch allocation failed: %s", what);
+  };
+  if (hipMalloc(&b->d_tab, tab_bytes) != hipSuccess) return fail(POYHIP_ENOMEM, "tables");
+  char* p = static_cast<char*>(b->d_tab);
+  b->d_dm_off = reinterpret_cast<int64_t*>(p); p += 8 * (size_t)n;
+  b->d_stab_off = reinterpret_cast<int64_t*>(p); p += 8 * (size_t)n;
+  b->d_out_off = reinterpret_cast<int64_t*>(p); p += 8 * (size_t)n;
+  b->d_la = reinterpret_cast<int*>(p); p += 4 * (size_t)n;
+  b->d_lb = reinterpret_cast<int*>(p); p += 4 * (size_t)n;
+  b->d_sw = reinterpret_cast<int*>(p); p += 4 * (size_t)n;
+  b->d_dw = reinterpret_cast<int*>(p); p += 4 * (size_t)n;
+  b->d_dwin = reinterpret_cast<int*>(p); p += 4 * (size_t)n;
+  b->d_order = reinterpret_cast<int*>(p); p += 4 * (size_t)n;
+  if (hipMalloc(&b->d_res, sizeof(PairResult) * (size_t)n) != hipSuccess)
+    return fail(POYHIP_ENOMEM, "results");
+  if (hipMalloc(&b->d_counters, sizeof(int) * (b->chunks.size() + 1)) != hipSuccess)
+    return fail(POYHIP_ENOMEM, "counters");
+  if (op_has_dm(op)) {
+    if (hipMalloc(&b->d_dm, (size_t)b->dm_cap + 256) != hipSuccess)
+      return fail(POYHIP_ENOMEM, "direction store");
+    if (hipMalloc(&b->d_stab, sizeof(long long) * (size_t)(so + 1)) != hipSuccess)
+      return fail(POYHIP_ENOMEM, "stripe tables");
+  }
+  if (hipMalloc(&b->d_ws, sizeof(int) * (size_t)b->ws_stride * b->grid) != hipSuccess)
+    return fail(POYHIP_ENOMEM, "workspace");
+  std::vector<int> dwin(n, 0);
+  if (deltaw) {
+    b->has_deltaw = 1;
+    for (int k = 0; k < n; k++) dwin[k] = deltaw[k];
+  }
+  hipStream_t st = c->stream;
+  bool ok = true;
+  ok &= hipMemcpyAsync(b->d_dm_off, b->dm_off.data(), 8 * (size_t)n, hipMemcpyHostToDevice, st) == hipSuccess;
+  ok &= hipMemcpyAsync(b->d_stab_off, b->stab_off.data(), 8 * (size_t)n, hipMemcpyHostToDevice, st) == hipSuccess;
+  ok &= hipMemcpyAsync(b->d_out_off, b->out_off.data(), 8 * (size_t)n, hipMemcpyHostToDevice, st) == hipSuccess;
+  ok &= hipMemcpyAsync(b->d_la, b->la.data(), 4 * (size_t)n, hipMemcpyHostToDevice, st) == hipSuccess;
+  ok &= hipMemcpyAsync(b->d_lb, b->lb.data(), 4 * (size_t)n, hipMemcpyHostToDevice, st) == hipSuccess;
+  ok &= hipMemcpyAsync(b->d_sw, b->swapped.data(), 4 * (size_t)n, hipMemcpyHostToDevice, st) == hipSuccess;
+  ok &= hipMemcpyAsync(b->d_dwin, dwin.data(), 4 * (size_t)n, hipMemcpyHostToDevice, st) == hipSuccess;
+  ok &= hipMemcpyAsync(b->d_order, b->order.data(), 4 * (size_t)n, hipMemcpyHostToDevice, st) == hipSuccess;
+  ok &= hipStreamSynchronize(st) == hipSuccess;
+  if (!ok) return fail(POYHIP_EDEVICE, "upload");
+  b->ev.resize(2 * b->chunks.size() + 2);
+  for (auto& e : b->ev)
+    if (hipEventCreate(&e) != hipSuccess) return fail(POYHIP_EDEVICE, "events");
+  (void)rc;
+  *out = b;
+  return POYHIP_OK;
+}
+
+int poyhip_batch_destroy(poyhip_batch* b) {
+  if (!b) return POYHIP_EINVAL;
+  (void)hipSetDevice(b->ctx->device);
+  (void)hipStreamSynchronize(b->ctx->stream);
+  batch_free_device(b);
+  delete b;
+  return POYHIP_OK;
+}
+
+size_t poyhip_batch_out_bytes(const poyhip_batch* b) { return b ? b->out_bytes : 0; }
+
+int64_t poyhip_batch_out_offset(const poyhip_batch* b, int k) {
+  if (!b || k < 0 || k >= b->n) return -1;
+  return b->out_off[k];
+}
+
+int poyhip_batch_out_cap(const poyhip_batch* b, int k) {
+  if (!b || k < 0 || k >= b->n) return -1;
+  if (b->op == POYHIP_OP_ALIGN_AFFINE) return b->la[k] + b->lb[k] + 2;
+  if (b->op == POYHIP_OP_ALIGN_2D) return (int)lin_out_cap(b->la[k], b->lb[k]);
+  return 0;
+}
+
+int poyhip_batch_use_graph(poyhip_batch* b, int enable) {
+  if (!b) return POYHIP_EINVAL;
+  b->use_graph = enable != 0;
+  if (!b->use_graph && b->gexec) {
+    (void)hipGraphExecDestroy(b->gexec);
+    b->gexec = nullptr;
+  }
+  return POYHIP_OK;
+}
+
+int poyhip_batch_run(poyhip_batch* b, const uint8_t* d_blob, const int64_t* d_off1,
+                     const int64_t* d_off2, int* d_cost, uint8_t* d_out, int* d_outlen, int sync) {
+  if (!b || !d_blob || !d_off1 || !d_off2 || !d_cost) return POYHIP_EINVAL;
+  poyhip_ctx* c = b->ctx;
+  if (op_has_dm(b->op) && (!d_out || !d_outlen))
+    return set_err(c, POYHIP_EINVAL, "align batches need output buffers");
+  (void)hipSetDevice(c->device);
+  hipStream_t st = c->stream;
+  if (b->use_graph) {
+    const void* key[6] = {d_blob, d_off1, d_off2, d_cost, d_out, d_outlen};
+    if (b->gexec && memcmp(key, b->g_key, sizeof key) != 0) {
+      (void)hipGraphExecDestroy(b->gexec);
+      b->gexec = nullptr;
+    }
+    if (!b->gexec) {
+      hipGraph_t g;
+      HIPCHK(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      int rc = batch_enqueue(b, d_blob, d_off1, d_off2, d_cost, d_out, d_outlen, false);
+      hipError_t e = hipStreamEndCapture(st, &g);
+      if (rc) return rc;
+      if (e != hipSuccess) return set_err(c, POYHIP_EDEVICE, "graph capture: %s", hipGetErrorString(e));
+      e = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess) {
+        b->gexec = nullptr;
+        return set_err(c, POYHIP_EDEVICE, "graph instantiate: %s", hipGetErrorString(e));
+      }
+      memcpy(b->g_key, key, sizeof key);
+    }
+    HIPCHK(c, hipGraphLaunch(b->gexec, st));
+    b->fill_ms = -1.f;
+  } else {
+    int rc = batch_enqueue(b, d_blob, d_off1, d_off2, d_cost, d_out, d_outlen, true);
+    if (rc) return rc;
+  }
+  if (sync) {
+    HIPCHK(c, hipStreamSynchronize(st));
+    if (!b->use_graph) {
+      float tot = 0.f;
+      for (size_t k = 0; k < b->chunks.size(); k++) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, b->ev[2 * k], b->ev[2 * k + 1]) == hipSuccess) tot += ms;
+      }
+      b->fill_ms = tot;
+      float all = 0.f;
+      const size_t nch = b->chunks.size();
+      if (hipEventElapsedTime(&all, b->ev[2 * nch], b->ev[2 * nch + 1]) == hipSuccess)
+        b->total_ms = all;
+    }
+  }
+  return POYHIP_OK;
+}
+
+int poyhip_batch_stats(poyhip_batch* b, int64_t* cells, int* rounds, int* newk) {
+  if (!b) return POYHIP_EINVAL;
+  poyhip_ctx* c = b->ctx;
+  (void)hipSetDevice(c->device);
+  std::vector<PairResult> r(b->n);
+  HIPCHK(c, hipMemcpyAsync(r.data(), b->d_res, sizeof(PairResult) * (size_t)b->n,
+                           hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int k = 0; k < b->n; k++) {
+    if (cells) cells[k] = r[k].cells;
+    if (rounds) rounds[k] = r[k].rounds;
+    if (newk) newk[k] = r[k].newk;
+  }
+  return POYHIP_OK;
+}
+
+float poyhip_batch_last_fill_ms(const poyhip_batch* b) { return b ? b->fill_ms : -1.f; }
+float poyhip_batch_last_total_ms(const poyhip_batch* b) { return b ? b->total_ms : -1.f; }
+
+}  // extern "C"

@@ -1,0 +1,111 @@
+// Device-side definitions shared by the POY DO alignment kernels (gfx950).
+//
+// Sequences are 5-bit nucleotide bitsets (A=1 C=2 G=4 T=8 gap=16) with the
+// leading gap at index 0, as POY stores them (src/fasta.ml:148).
+//
+// Work decomposition used by every 2D fill kernel: one 64-lane wavefront per
+// pair.  A pair's DP rows are cut into stripes of 64 rows; lane t owns row
+// i0+t of the stripe and the wavefront sweeps the stripe's band columns as a
+// systolic pipeline: at step s lane t evaluates column jlo+s-t, so the cell
+// above (row i-1, same column) was produced by lane t-1 one step earlier and
+// arrives through a single DPP wave_shr:1 move; the up-left cell is the
+// value received on the previous step.  Lane 0 takes its "above" row from a
+// boundary row written by lane 63 of the previous stripe (or generates POY's
+// row 0).  Direction information is stored per stripe as
+// [step/pack][lane] words, so every store of the wavefront is one coalesced
+// 256-byte transaction.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace poyhip {
+
+constexpr int kHigh = 1000000;  // HIGH_NUM, src/algn.c:37
+constexpr int kGap = 16;
+constexpr int kWave = 64;
+
+// 2D flags (src/matrices.h:22-27)
+constexpr int kAlign = 1, kInsert = 2, kDelete = 4;
+
+// Cost model resident in device memory; staged into LDS by each kernel.
+struct CmDev {
+  int cost[1024];     // cost[(a<<5)+b] for 5-bit codes a, b
+  int worst[1024];
+  int prepend[32];
+  int tail[32];
+  uint8_t median[1024];
+  int gap_open;
+  int delta;          // smallest positive entry of the full 2*32*32 table
+  int cost_model;     // 0 linear, 1 affine
+  int combinations;
+};
+
+// Descriptor arrays of a batch (structure of arrays, device memory).  The
+// DP rows always come from the shorter sequence: `swapped` says that the
+// caller's second sequence is the shorter one.  `prio` is the swap flag of
+// the reference's backtraces (insert/delete and horizontal/vertical tie
+// order); batches set it equal to `swapped`, the per-pair drop-ins take it
+// from the caller as the reference stubs do.
+struct PairTable {
+  const uint8_t* blob;    // all sequences
+  const int64_t* off1;    // caller's first sequence
+  const int64_t* off2;    // caller's second sequence
+  const int* len_a;       // rows (shorter)
+  const int* len_b;       // columns
+  const int* swapped;
+  const int* prio;
+  const int* deltawh;     // linear: Align.cost_2 band hint
+  const int64_t* dm_off;  // byte offset of the pair's direction store
+  const int64_t* stab_off;// offset of the pair's stripe table (int64 units)
+  const int64_t* out_off; // byte offset of the pair's output block
+  const int* order;       // pairs of this launch, largest first
+  int n;                  // entries in `order`
+  int npairs;             // pairs in the tables (backtrace grid)
+};
+
+__device__ __forceinline__ int64_t rows_off(const PairTable& t, int pid) {
+  return t.swapped[pid] ? t.off2[pid] : t.off1[pid];
+}
+__device__ __forceinline__ int64_t cols_off(const PairTable& t, int pid) {
+  return t.swapped[pid] ? t.off1[pid] : t.off2[pid];
+}
+
+struct PairResult {
+  int cost;
+  int gap_num;
+  int newk;       // band half-width of the final round (-1: no DP rows)
+  int rounds;
+  int full;       // linear: 1 when the full plane was filled
+  int pad;
+  long long cells;  // cells evaluated over all rounds
+};
+
+// Wavefront shift right by one lane; lane 0 receives `lane0`.
+__device__ __forceinline__ int wave_shr1(int v, int lane0) {
+  return __builtin_amdgcn_update_dpp(lane0, v, 0x138, 0xF, 0xF, false);
+}
+
+__device__ __forceinline__ int imax(int a, int b) { return a > b ? a : b; }
+__device__ __forceinline__ int imin(int a, int b) { return a < b ? a : b; }
+
+// HAS_GAP_OPENING (src/algn.c:1240-1253) with gap_startNO = 0.
+__device__ __forceinline__ int has_gap_opening(int idx, int prev, int curr, int go) {
+  if (idx == 1 && (kGap & curr)) return 0;
+  if (idx > 1 && !(kGap & prev) && (kGap & curr)) return 0;
+  return go;
+}
+
+// choose_dir (src/algn.c:1594-1624): 0 vertical, 1 horizontal, 2 diagonal,
+// 3 align.
+__device__ __forceinline__ int choose_dir(int has_diag, int has_vert, int has_hrzn, int sw) {
+  if (!sw) {
+    if (has_vert) return 0;
+    if (has_hrzn) return 1;
+  } else {
+    if (has_hrzn) return 1;
+    if (has_vert) return 0;
+  }
+  return has_diag ? 2 : 3;
+}
+
+}  // namespace poyhip

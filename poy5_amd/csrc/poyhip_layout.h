@@ -1,0 +1,41 @@
+// Sizes of the per-pair device regions, shared by host planning code and the
+// kernels.  La/Lb are sequence lengths including the leading gap; La <= Lb.
+#pragma once
+#include <stdint.h>
+
+#ifdef __HIPCC__
+#define POYHIP_HD __host__ __device__
+#else
+#define POYHIP_HD
+#endif
+
+namespace poyhip {
+
+POYHIP_HD inline int n_stripes(int La) { return (La - 1 + 63) >> 6; }
+
+// ---- banded affine ------------------------------------------------------
+// per-wavefront workspace ints: row-0 block-diagonal buffer (Lb) and two
+// boundary rows of five arrays (V, CB, H, ED, gap counts)
+POYHIP_HD inline int64_t aff_ws_ints(int Lb) { return (int64_t)11 * Lb + 16; }
+// direction bytes: one byte per (step, lane), at most Lb+63 steps a stripe
+POYHIP_HD inline int64_t aff_dm_bytes(int La, int Lb) {
+  return (int64_t)n_stripes(La) * (((int64_t)(Lb - 1) + 64 + 3) >> 2) * 256;
+}
+// outputs: resi, resj, median, medianwg, each La+Lb+2 bytes
+POYHIP_HD inline int64_t aff_out_bytes(int La, int Lb) { return 4 * (int64_t)(La + Lb + 2); }
+
+// ---- full affine cost (no directions) -----------------------------------
+POYHIP_HD inline int64_t affcost_ws_ints(int Lb) { return (int64_t)8 * Lb + 16; }
+
+// ---- non-affine ---------------------------------------------------------
+// per-wavefront workspace: two boundary rows of (cost, delete count)
+POYHIP_HD inline int64_t lin_ws_ints(int Lb) { return (int64_t)4 * Lb + 16; }
+// 2-bit moves: 16 steps per 32-bit word per lane
+POYHIP_HD inline int64_t lin_dm_bytes(int La, int Lb) {
+  return (int64_t)n_stripes(La) * (((int64_t)Lb + 63 + 15) >> 4) * 256;
+}
+// outputs: r1, r2 (La+Lb each), ancestor median (La+Lb+1), median with gaps
+POYHIP_HD inline int64_t lin_out_cap(int La, int Lb) { return (int64_t)La + Lb + 1; }
+POYHIP_HD inline int64_t lin_out_bytes(int La, int Lb) { return 4 * lin_out_cap(La, Lb); }
+
+}  // namespace poyhip

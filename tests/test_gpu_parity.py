@@ -1,0 +1,159 @@
+"""GPU parity: every drop-in and batch entry point against the CPU oracle,
+bit-exact (costs, aligned sequences, medians are integer/byte data)."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from poy5_amd import cost_matrix, workloads
+from poy5_amd.batch import run_pairs
+from poy5_amd.sequence import Align, Context
+
+pytestmark = pytest.mark.gpu
+
+PLAIN = np.array([1, 2, 4, 8], np.uint8)
+IUPAC = np.array([3, 5, 9, 6, 10, 12, 7, 11, 13, 14, 15], np.uint8)
+GAPPY = np.array([17, 18, 20, 24, 31, 19, 28], np.uint8)  # codes carrying the gap bit
+
+
+def rand_seq(rng, n, gappy=0.0, iupac=0.05):
+    s = rng.choice(PLAIN, n)
+    m = rng.random(n)
+    s = np.where(m < iupac, rng.choice(IUPAC, n), s)
+    s = np.where(m > 1 - gappy, rng.choice(GAPPY, n), s)
+    return np.concatenate([[16], s]).astype(np.uint8)
+
+
+def mutate(rng, s, d):
+    out = [16]
+    for c in s[1:]:
+        u = rng.random()
+        if u < d / 3:
+            continue
+        if u < 2 * d / 3:
+            out.append(int(rng.choice(PLAIN)))
+        out.append(int(rng.choice(PLAIN)) if rng.random() < d else int(c))
+    return np.array(out, np.uint8)
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = Context(0)
+    yield c
+    c.close()
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def pair_cases(seed, count, maxlen, gappy=0.0):
+    rng = np.random.default_rng(seed)
+    cases = []
+    for k in range(count):
+        la = int(rng.integers(0, maxlen))
+        root = rand_seq(rng, la, gappy)
+        a = mutate(rng, root, rng.uniform(0.02, 0.3))
+        b = mutate(rng, root, rng.uniform(0.02, 0.3))
+        cases.append((a, b))
+    # edge cases: leading gap only, equal, single symbol
+    cases += [(np.array([16], np.uint8), np.array([16], np.uint8)),
+              (np.array([16], np.uint8), rand_seq(rng, 5)),
+              (rand_seq(rng, 1), rand_seq(rng, 1)),
+              (rand_seq(rng, 7), rand_seq(rng, 7))]
+    return cases
+
+
+@pytest.mark.parametrize("gappy", [0.0, 0.15])
+def test_align_affine_3_dropin(ctx, gappy):
+    say(f"test_align_affine_3_dropin gappy={gappy}")
+    cm = cost_matrix.of_transformations_and_gaps(1, 1, 3)
+    ctx.set_cm(cm)
+    ocm = oracle.cm_dna(1, 1, 3)
+    al = Align(ctx)
+    for k, (a, b) in enumerate(pair_cases(11 + int(gappy * 100), 60, 400, gappy)):
+        if len(a) > len(b):
+            a, b = b, a
+        for sw in (0, 1):
+            m, ra, rb, c, mwg = al.c_align_affine_3(a, b, sw)
+            ref = oracle.align_affine_3(a, b, ocm, sw)
+            assert c == ref["cost"], (k, sw, c, ref["cost"])
+            assert np.array_equal(ra, ref["resi"]), (k, sw)
+            assert np.array_equal(rb, ref["resj"]), (k, sw)
+            assert np.array_equal(m, ref["median"]), (k, sw)
+            assert np.array_equal(mwg, ref["medianwg"]), (k, sw)
+
+
+def test_cost_affine_3_dropin(ctx):
+    say("test_cost_affine_3_dropin")
+    ctx.set_cm(cost_matrix.of_transformations_and_gaps(1, 1, 3))
+    ocm = oracle.cm_dna(1, 1, 3)
+    al = Align(ctx)
+    for k, (a, b) in enumerate(pair_cases(5, 40, 300, 0.1)):
+        assert al.cost_2_affine(a, b) == oracle.cost_affine_3(a, b, ocm), k
+
+
+@pytest.mark.parametrize("tcm", [(1, 2), (1, 1)])
+def test_align_2_linear(ctx, tcm):
+    say(f"test_align_2_linear tcm={tcm}")
+    ctx.set_cm(cost_matrix.of_transformations_and_gaps(*tcm))
+    ocm = oracle.cm_dna(*tcm)
+    al = Align(ctx)
+    for k, (a, b) in enumerate(pair_cases(21 + tcm[1], 60, 500, 0.1)):
+        r1, r2, c = al.align_2(a, b)
+        o1, o2, oc, _ = oracle.align_2(a, b, ocm)
+        assert c == oc, (k, c, oc)
+        assert np.array_equal(r1, o1) and np.array_equal(r2, o2), k
+        # the deltaw form used by DOS.distance
+        assert al.cost_2(a, b, deltaw=8) == oracle.cost_2(a, b, ocm, deltaw=8)[0], k
+
+
+def test_medians_dropin(ctx):
+    say("test_medians_dropin")
+    for model in [(1, 2, None), (1, 1, 3)]:
+        ctx.set_cm(cost_matrix.of_transformations_and_gaps(*model))
+        ocm = oracle.cm_dna(*model)
+        al = Align(ctx)
+        for k, (a, b) in enumerate(pair_cases(31, 20, 200, 0.2)):
+            o1, o2, _, _ = oracle.align_2(a, b, oracle.cm_dna(1, 2))
+            assert np.array_equal(al.ancestor_2(o1, o2), oracle.ancestor_2(o1, o2, ocm)), k
+            assert np.array_equal(al.median_2(o1, o2), oracle.median_2_no_gaps(o1, o2, ocm)), k
+            assert np.array_equal(al.median_2_with_gaps(o1, o2),
+                                  oracle.median_2_with_gaps(o1, o2, ocm)), k
+            assert al.max_cost_2(o1, o2) == oracle.worst_2(o1, o2, ocm), k
+
+
+def test_batch_affine_config2_sample(ctx):
+    say("test_batch_affine_config2_sample")
+    ctx.set_cm(cost_matrix.of_transformations_and_gaps(1, 1, 3))
+    ocm = oracle.cm_dna(1, 1, 3)
+    pairs = workloads.pairs_list(2, 96)
+    pairs = [p for p in pairs if len(p[0]) * len(p[1]) < 1_200_000][:48]
+    costs, outs = run_pairs(ctx, "align_affine", pairs)
+    for k, (a, b) in enumerate(pairs):
+        sw = len(a) > len(b)
+        si, sj = (b, a) if sw else (a, b)
+        ref = oracle.align_affine_3(si, sj, ocm, int(sw))
+        ra, rb = (ref["resj"], ref["resi"]) if sw else (ref["resi"], ref["resj"])
+        assert costs[k] == ref["cost"], (k, costs[k], ref["cost"])
+        assert np.array_equal(outs[k][0], ra) and np.array_equal(outs[k][1], rb), k
+        assert np.array_equal(outs[k][2], ref["median"]), k
+        assert np.array_equal(outs[k][3], ref["medianwg"]), k
+
+
+def test_batch_linear_config1_sample(ctx):
+    say("test_batch_linear_config1_sample")
+    ctx.set_cm(cost_matrix.of_transformations_and_gaps(1, 2))
+    ocm = oracle.cm_dna(1, 2)
+    pairs = workloads.pairs_list(1, 32)
+    costs, outs = run_pairs(ctx, "align_2d", pairs)
+    for k, (a, b) in enumerate(pairs):
+        o1, o2, oc, _ = oracle.align_2(a, b, ocm)
+        assert costs[k] == oc, (k, costs[k], oc)
+        assert np.array_equal(outs[k][0], o1) and np.array_equal(outs[k][1], o2), k
+        assert np.array_equal(outs[k][2], oracle.ancestor_2(o1, o2, ocm)), k
+        assert np.array_equal(outs[k][3], oracle.median_2_with_gaps(o1, o2, ocm)), k
+    costs2, _ = run_pairs(ctx, "cost_2d", pairs)
+    assert np.array_equal(costs2, costs)
