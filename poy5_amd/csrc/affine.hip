@@ -45,9 +45,7 @@ __global__ __launch_bounds__(64) void k_affine_fill(AffFillArgs A) {
   const int delta = A.cm->delta;
 
   for (;;) {
-    int q = 0;
-    if (lane == 0) q = atomicAdd(A.counter, 1);
-    q = __builtin_amdgcn_readfirstlane(q);
+    const int q = wave_grab(A.counter);
     if (q >= A.pt.n) break;
     const int pid = A.pt.order[q];
     const uint8_t* __restrict__ sa = A.pt.blob + rows_off(A.pt, pid);
@@ -284,8 +282,9 @@ __global__ __launch_bounds__(64) void k_affine_fill(AffFillArgs A) {
         __threadfence_block();
       }
       const int src = (leni - 1) & 63;
-      cost = __shfl(rc, src);
-      gap_num = __shfl(rgn, src);
+      // uniform by construction; the broadcast keeps the round loop scalar
+      cost = __builtin_amdgcn_readfirstlane(__shfl(rc, src));
+      gap_num = __builtin_amdgcn_readfirstlane(__shfl(rgn, src));
       rounds++;
       const int newp = (2 * T - D) / 2;
       if (gap_num < p || newp - lenj + 1 >= 0) break;
@@ -446,9 +445,7 @@ __global__ __launch_bounds__(64) void k_affine_cost_full(AffCostArgs A) {
   __syncthreads();
   const int go = A.cm->gap_open;
   for (;;) {
-    int q = 0;
-    if (lane == 0) q = atomicAdd(A.counter, 1);
-    q = __builtin_amdgcn_readfirstlane(q);
+    const int q = wave_grab(A.counter);
     if (q >= A.pt.n) break;
     const int pid = A.pt.order[q];
     const uint8_t* __restrict__ sa = A.pt.blob + rows_off(A.pt, pid);
@@ -555,7 +552,7 @@ __global__ __launch_bounds__(64) void k_affine_cost_full(AffCostArgs A) {
       }
       __threadfence_block();
     }
-    const int cost = __shfl(rc, (leni - 1) & 63);
+    const int cost = __builtin_amdgcn_readfirstlane(__shfl(rc, (leni - 1) & 63));
     if (lane == 0) {
       PairResult r{};
       r.cost = cost;

@@ -50,9 +50,7 @@ __global__ __launch_bounds__(64) void k_linear_fill(LinFillArgs A) {
   const int delta = A.cm->delta;
 
   for (;;) {
-    int q = 0;
-    if (lane == 0) q = atomicAdd(A.counter, 1);
-    q = __builtin_amdgcn_readfirstlane(q);
+    const int q = wave_grab(A.counter);
     if (q >= A.pt.n) break;
     const int pid = A.pt.order[q];
     const uint8_t* __restrict__ sa = A.pt.blob + rows_off(A.pt, pid);
@@ -198,10 +196,11 @@ __global__ __launch_bounds__(64) void k_linear_fill(LinFillArgs A) {
       }
       if (L1 > 1) {
         const int src = (L1 - 2) & 63;
-        cost = __shfl(rc, src);
-        gap_num = __shfl(rdc, src) + Dd;
+        // uniform by construction; the broadcast keeps the round loop scalar
+        cost = __builtin_amdgcn_readfirstlane(__shfl(rc, src));
+        gap_num = __builtin_amdgcn_readfirstlane(__shfl(rdc, src)) + Dd;
       } else {
-        cost = rc;
+        cost = __builtin_amdgcn_readfirstlane(rc);
         gap_num = L2 - 1;
       }
       rounds++;

@@ -85,6 +85,16 @@ __device__ __forceinline__ int wave_shr1(int v, int lane0) {
   return __builtin_amdgcn_update_dpp(lane0, v, 0x138, 0xF, 0xF, false);
 }
 
+// Next work item of a persistent wavefront.  Every lane issues the atomic
+// (lane 0 adds 1, the others 0) so the grab sits in uniform control flow and
+// the loop around it keeps a single header: a lane-0-only branch before the
+// broadcast let the compiler split the loop so that lanes 1..63 never
+// returned to the atomic.  Lane 0's old value is the index.
+__device__ __forceinline__ int wave_grab(int* counter) {
+  const int old = atomicAdd(counter, (threadIdx.x & 63) == 0 ? 1 : 0);
+  return __builtin_amdgcn_readfirstlane(old);
+}
+
 __device__ __forceinline__ int imax(int a, int b) { return a > b ? a : b; }
 __device__ __forceinline__ int imin(int a, int b) { return a < b ? a : b; }
 
