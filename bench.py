@@ -1,0 +1,244 @@
+#!/usr/bin/env python3
+"""Benchmark: batched uneven affine-gap DO alignment on one MI355X.
+
+Workload (config 2): 65,536 DNA pairs, root lengths log-uniform 50-5,000,
+per-pair divergence U[2%,25%] at the 10:3 substitution:indel ratio, 5% IUPAC
+codes; affine costs substitution 1, gap extension 1, gap opening 3.  One
+step = POY's align_affine_3 (banded fill with Ukkonen doubling, backtrace,
+median and median-with-gaps) over every pair, inputs already resident in HBM.
+
+Printed JSON (one line): GCUPS on full cells (sum of L1*L2, leading gaps
+included) per second of the timed steps, the fill kernel's VALU roofline,
+the plain per-call GPU path (one launch per pair through the drop-in call)
+on a sample, and the CPU oracle on a bounded sample of host cores.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "GCUPS on uneven affine-gap pair batches vs plain per-call; p50/p99 latency"
+OPS_PER_CELL = 35          # int32 adds+compares per affine cell with backtrace state
+VALU_PEAK_TOPS = 39.3      # 256 CU x 64 lanes x 2.4 GHz int32 VALU ops/s (1e12)
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--pairs", type=int, default=65536)
+    ap.add_argument("--plain-sample", type=int, default=512)
+    ap.add_argument("--cpu-stride", type=int, default=16,
+                    help="CPU baseline runs every N-th pair")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--graph", type=int, default=1)
+    ap.add_argument("--dm-budget-gb", type=float, default=0.0)
+    return ap.parse_args()
+
+
+_CPU_DATA = None  # (blob, off, ln), inherited by the forked workers
+
+
+def cpu_worker(idx):
+    """One host process of the CPU baseline: the oracle over its share."""
+    blob, off, ln = _CPU_DATA
+    import numpy as np
+    import oracle
+    L = oracle.lib()
+    cm = oracle.cm_dna(1, 1, 3)
+    off1 = np.ascontiguousarray(off[0::2][idx])
+    off2 = np.ascontiguousarray(off[1::2][idx])
+    l1 = np.ascontiguousarray(ln[0::2][idx]).astype(np.int32)
+    l2 = np.ascontiguousarray(ln[1::2][idx]).astype(np.int32)
+    costs = np.zeros(len(idx), np.int32)
+    cells = C.c_int64()
+    t0 = time.perf_counter()
+    L.orc_batch_align_affine(len(idx), blob.ctypes.data_as(C.POINTER(C.c_uint8)),
+                             off1.ctypes.data_as(C.POINTER(C.c_int64)),
+                             l1.ctypes.data_as(C.POINTER(C.c_int)),
+                             off2.ctypes.data_as(C.POINTER(C.c_int64)),
+                             l2.ctypes.data_as(C.POINTER(C.c_int)), C.byref(cm),
+                             costs.ctypes.data_as(C.POINTER(C.c_int)), C.byref(cells))
+    return time.perf_counter() - t0, costs, idx
+
+
+def cpu_baseline(blob, off, ln, stride, gpu_costs):
+    """The oracle (a C restatement of POY's algn.c path) on every stride-th
+    pair, one process per host core (16 at most), LPT-partitioned."""
+    import multiprocessing as mp
+    import numpy as np
+    n = len(ln) // 2
+    sample = np.arange(0, n, stride)
+    work = ln[0::2][sample].astype(np.float64) * ln[1::2][sample]
+    cores = max(1, min(16, os.cpu_count() or 1))
+    parts = [[] for _ in range(cores)]
+    load = [0.0] * cores
+    for k in np.argsort(-work):
+        p = int(np.argmin(load))
+        parts[p].append(sample[k])
+        load[p] += work[k]
+    global _CPU_DATA
+    _CPU_DATA = (blob, off, ln)
+    jobs = [np.array(p, np.int64) for p in parts if p]
+    ctx = mp.get_context("fork")
+    t0 = time.perf_counter()
+    with ctx.Pool(len(jobs)) as pool:
+        res = pool.map(cpu_worker, jobs)
+    wall = time.perf_counter() - t0
+    match = all(np.array_equal(c, gpu_costs[i]) for _, c, i in res)
+    cells = float(work.sum())
+    return {"value": cells / wall / 1e9, "unit": "GCUPS", "cores": len(jobs), "kind": "port",
+            "sample": f"every {stride}th pair of the workload ({len(sample)} pairs, "
+                      f"{cells:.3e} full cells), oracle/oracle.c compiled -O2, "
+                      f"{len(jobs)} processes, wall {wall:.2f} s",
+            "costs_match_gpu": bool(match)}
+
+
+def plain_per_call(ctx, al, blob, off, ln, sample_n, seed=1):
+    """One drop-in call (algn_CAML_align_affine_3) per pair: upload, fill
+    launch, backtrace launch, download, synchronise."""
+    import numpy as np
+    n = len(ln) // 2
+    rng = np.random.default_rng(seed)
+    idx = rng.choice(n, size=min(sample_n, n), replace=False)
+    lat = []
+    cells = 0.0
+    seqs = []
+    for k in idx:
+        a = blob[off[2 * k]: off[2 * k] + ln[2 * k]]
+        b = blob[off[2 * k + 1]: off[2 * k + 1] + ln[2 * k + 1]]
+        seqs.append((a, b))
+    al.align_affine_3(*seqs[0])  # first-call setup outside the timing
+    t0 = time.perf_counter()
+    for a, b in seqs:
+        t1 = time.perf_counter()
+        al.align_affine_3(a, b)
+        lat.append((time.perf_counter() - t1) * 1e3)
+        cells += float(len(a)) * len(b)
+    wall = time.perf_counter() - t0
+    lat = np.array(lat)
+    return {"gcups": cells / wall / 1e9, "pairs": int(len(idx)), "cells": cells,
+            "wall_s": wall, "p50_ms": float(np.percentile(lat, 50)),
+            "p99_ms": float(np.percentile(lat, 99))}
+
+
+def load_traffic():
+    p = os.path.join(ROOT, "profiles", "affine_fill_config2_traffic.json")
+    if os.path.exists(p):
+        with open(p) as f:
+            return json.load(f)
+    return None
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build runs on a single MI355X and "
+              f"has no multi-GPU path (asked for --gpus {args.gpus}).", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    from poy5_amd import cost_matrix, workloads
+    from poy5_amd.batch import Batch
+    from poy5_amd.sequence import Align, Context
+
+    print(f"[bench] generating config-2 workload ({args.pairs} pairs)", file=sys.stderr, flush=True)
+    blob, off, ln = workloads.gen_pairs(2, args.pairs)
+    len1 = ln[0::2].astype(np.int32)
+    len2 = ln[1::2].astype(np.int32)
+    cells_full = float(np.sum(len1.astype(np.float64) * len2))
+
+    ctx = Context(0)
+    ctx.set_cm(cost_matrix.of_transformations_and_gaps(1, 1, 3))
+    budget = int(args.dm_budget_gb * (1 << 30))
+    b = Batch(ctx, "align_affine", len1, len2, dm_budget=budget)
+    dev = torch.device("cuda:0")
+    d_blob = torch.from_numpy(blob).to(dev)
+    d_off1 = torch.from_numpy(np.ascontiguousarray(off[0::2])).to(dev)
+    d_off2 = torch.from_numpy(np.ascontiguousarray(off[1::2])).to(dev)
+    cost, out, outlen = b.alloc_outputs(dev)
+    torch.cuda.synchronize()
+
+    # one instrumented run: fill-kernel duration from HIP events on the
+    # library's stream, plus per-pair band statistics
+    print("[bench] instrumented run", file=sys.stderr, flush=True)
+    b.run(d_blob, d_off1, d_off2, cost, out, outlen, sync=True)
+    fill_ms = b.fill_ms()
+    total_ms = b.total_ms()
+    cells_band, rounds, _ = b.stats()
+    cells_computed = float(cells_band.sum())
+    gpu_costs = cost.cpu().numpy()
+
+    if args.graph:
+        b.use_graph(True)
+    for _ in range(args.warmup):
+        b.run(d_blob, d_off1, d_off2, cost, out, outlen, sync=True)
+    torch.cuda.synchronize()
+    ctx.chk(ctx._lib.poyhip_sync(ctx.h))
+    print(f"[bench] timing {args.steps} steps", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        b.run(d_blob, d_off1, d_off2, cost, out, outlen, sync=False)
+    ctx.chk(ctx._lib.poyhip_sync(ctx.h))
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ms_per_step = dt * 1e3 / args.steps
+    gcups = cells_full / (ms_per_step / 1e3) / 1e9
+    assert np.array_equal(cost.cpu().numpy(), gpu_costs)
+
+    ops = cells_computed * OPS_PER_CELL
+    achieved = ops / (fill_ms / 1e3) / 1e12
+    traffic = load_traffic()
+    roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": VALU_PEAK_TOPS,
+                "unit": "TOP/s", "frac": round(achieved / VALU_PEAK_TOPS, 4),
+                "traffic": traffic.get("bytes_per_launch") if traffic else None,
+                "kernel": "k_affine_fill<true>",
+                "ops_per_cell": OPS_PER_CELL, "cells_per_launch": cells_computed,
+                "fill_ms": round(fill_ms, 3), "run_ms": round(total_ms, 3)}
+    dm_bytes = cells_computed  # one direction byte per evaluated cell
+    roofline["hbm_secondary"] = {"algorithmic_bytes": dm_bytes + float(blob.size),
+                                 "achieved_GBs": round((dm_bytes + blob.size) / (fill_ms / 1e3) / 1e9, 1),
+                                 "peak_GBs": HBM_PEAK_GBS}
+
+    result = {
+        "metric": METRIC, "value": round(gcups, 2), "unit": "GCUPS", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32",
+        "data": "synthetic",
+        "config": {"workload": "config2: 65,536 uneven affine DNA pairs (log-uniform 50-5000, "
+                               "divergence 2-25%, sub 1 / ext 1 / open 3), align_affine_3 per pair",
+                   "pairs": int(args.pairs), "cells_full": cells_full,
+                   "cells_computed": cells_computed, "mean_rounds": float(rounds.mean()),
+                   "graph": bool(args.graph), "dm_budget_gb": args.dm_budget_gb},
+        "roofline": roofline,
+    }
+    if not args.no_plain:
+        print("[bench] plain per-call sample", file=sys.stderr, flush=True)
+        al = Align(ctx)
+        pl = plain_per_call(ctx, al, blob, off, ln, args.plain_sample)
+        result["plain_per_call"] = pl
+        result["batched_vs_plain"] = round(gcups / pl["gcups"], 2)
+        result["p50_ms"] = round(pl["p50_ms"], 3)
+        result["p99_ms"] = round(pl["p99_ms"], 3)
+    if not args.no_cpu:
+        print("[bench] CPU baseline", file=sys.stderr, flush=True)
+        result["cpu_baseline"] = cpu_baseline(blob, off, ln, args.cpu_stride, gpu_costs)
+    b.close()
+    ctx.close()
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()
