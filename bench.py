@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--graph", type=int, default=1)
     ap.add_argument("--dm-budget-gb", type=float, default=0.0)
+    ap.add_argument("--copy-calib-gb", type=float, default=0.0,
+                    help="also run one device-to-device copy of this size (counter calibration)")
     return ap.parse_args()
 
 
@@ -177,8 +179,9 @@ def main():
     b.run(d_blob, d_off1, d_off2, cost, out, outlen, sync=True)
     fill_ms = b.fill_ms()
     total_ms = b.total_ms()
-    cells_band, rounds, _ = b.stats()
+    cells_band, slots, rounds, _ = b.stats()
     cells_computed = float(cells_band.sum())
+    lane_slots = float(slots.sum())
     gpu_costs = cost.cpu().numpy()
 
     if args.graph:
@@ -220,7 +223,9 @@ def main():
         "config": {"workload": "config2: 65,536 uneven affine DNA pairs (log-uniform 50-5000, "
                                "divergence 2-25%, sub 1 / ext 1 / open 3), align_affine_3 per pair",
                    "pairs": int(args.pairs), "cells_full": cells_full,
-                   "cells_computed": cells_computed, "mean_rounds": float(rounds.mean()),
+                   "cells_computed": cells_computed, "lane_steps": lane_slots,
+                   "lane_occupancy": round(cells_computed / max(lane_slots, 1.0), 4),
+                   "mean_rounds": float(rounds.mean()),
                    "graph": bool(args.graph), "dm_budget_gb": args.dm_budget_gb},
         "roofline": roofline,
     }
@@ -235,6 +240,15 @@ def main():
     if not args.no_cpu:
         print("[bench] CPU baseline", file=sys.stderr, flush=True)
         result["cpu_baseline"] = cpu_baseline(blob, off, ln, args.cpu_stride, gpu_costs)
+    if args.copy_calib_gb > 0:
+        nb = int(args.copy_calib_gb * (1 << 30))
+        x = torch.ones(nb, dtype=torch.uint8, device=dev)
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        y.copy_(x)
+        torch.cuda.synchronize()
+        result["copy_calibration_bytes"] = nb
+        del x, y
     b.close()
     ctx.close()
     print(json.dumps(result), flush=True)

@@ -159,8 +159,10 @@ int poyhip_batch_run(poyhip_batch *b, const uint8_t *d_blob, const int64_t *d_of
                      const int64_t *d_off2, int *d_cost, uint8_t *d_out,
                      int *d_outlen, int sync);
 /* Per-pair statistics of the last run (host arrays of n; any may be NULL):
- * cells evaluated over all doubling rounds, rounds, final band half-width. */
-int poyhip_batch_stats(poyhip_batch *b, int64_t *cells, int *rounds, int *newk);
+ * cells evaluated over all doubling rounds, lane-steps the fill issued for
+ * them (cells plus idle lanes), rounds, final band half-width. */
+int poyhip_batch_stats(poyhip_batch *b, int64_t *cells, int64_t *slots, int *rounds,
+                       int *newk);
 /* Captures the batch's launches into a HIP graph and replays it on later
  * runs with the same device pointers. */
 int poyhip_batch_use_graph(poyhip_batch *b, int enable);

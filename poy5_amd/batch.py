@@ -100,12 +100,14 @@ class Batch:
 
     def stats(self):
         cells = np.zeros(self.n, np.int64)
+        slots = np.zeros(self.n, np.int64)
         rounds = np.zeros(self.n, np.int32)
         newk = np.zeros(self.n, np.int32)
         self.ctx.chk(self.L.poyhip_batch_stats(
             self.h, cells.ctypes.data_as(C.POINTER(C.c_int64)),
+            slots.ctypes.data_as(C.POINTER(C.c_int64)),
             rounds.ctypes.data_as(C.POINTER(C.c_int)), newk.ctypes.data_as(C.POINTER(C.c_int))))
-        return cells, rounds, newk
+        return cells, slots, rounds, newk
 
     def unpack(self, out_host: np.ndarray, outlen_host: np.ndarray, k: int):
         """Four output sequences of pair k (see poyhip_batch_out_offset)."""

@@ -83,6 +83,7 @@ __global__ __launch_bounds__(64) void k_affine_fill(AffFillArgs A) {
         r.full = 0;
         r.pad = 0;
         r.cells = 0;
+        r.slots = 0;
         A.res[pid] = r;
       }
       continue;
@@ -94,7 +95,7 @@ __global__ __launch_bounds__(64) void k_affine_fill(AffFillArgs A) {
 
     int T = (D + 1) * delta;
     int rounds = 0, newk = 0, cost = 0, gap_num = 0;
-    long long cells = 0;
+    long long cells = 0, slots = 0;
     for (;;) {
       const int p = (T - D) / 2;
       newk = p >= leni ? leni - 1 : p;
@@ -110,6 +111,7 @@ __global__ __launch_bounds__(64) void k_affine_fill(AffFillArgs A) {
         const int jhi = (ilast + D + newk <= lenj - 1) ? ilast + D + newk : lenj;
         const int nsteps = (jhi - jlo + 1) + (ilast - i0);
         if (kDm && lane == 0) stab[b] = dmpos;
+        slots += (long long)nsteps * 64;
 
         int ic = 0, ip = 0, si_ge = 0, si_go = 0, si_ng = 0, si_vext = 0;
         int my_lo = 1 << 30, my_hi = -1, my_rb = -1;
@@ -301,6 +303,7 @@ __global__ __launch_bounds__(64) void k_affine_fill(AffFillArgs A) {
       r.full = 0;
       r.pad = 0;
       r.cells = cells;
+      r.slots = slots;
       A.res[pid] = r;
     }
   }

@@ -65,7 +65,7 @@ __global__ __launch_bounds__(64) void k_linear_fill(LinFillArgs A) {
 
     int T = (Dd + 1) * delta;
     int rounds = 0, newk = 0, cost = 0, gap_num = 0;
-    long long cells = 0;
+    long long cells = 0, slots = 0;
     for (;;) {
       const int p = (T - Dd) / 2;
       newk = full ? L1 - 1 : (p >= L1 ? L1 - 1 : p);
@@ -90,6 +90,7 @@ __global__ __launch_bounds__(64) void k_linear_fill(LinFillArgs A) {
         const int jhi = full ? L2 - 1 : imin(ilast + Dd + newk, L2 - 1);
         const int nsteps = (jhi - jlo + 1) + (ilast - i0);
         if (kDm && lane == 0) stab[b] = dmpos;
+        slots += (long long)nsteps * 64;
         int s1i = 0, c = 0, tl = 0, col0 = 0, lo = 1 << 30, hi = -1;
         bool hasL = false, hasR = false;
         if (act) {
@@ -219,6 +220,7 @@ __global__ __launch_bounds__(64) void k_linear_fill(LinFillArgs A) {
       r.full = full ? 1 : 0;
       r.pad = 0;
       r.cells = cells;
+      r.slots = slots;
       A.res[pid] = r;
     }
   }

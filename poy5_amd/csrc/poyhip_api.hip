@@ -928,7 +928,7 @@ int poyhip_batch_run(poyhip_batch* b, const uint8_t* d_blob, const int64_t* d_of
   return POYHIP_OK;
 }
 
-int poyhip_batch_stats(poyhip_batch* b, int64_t* cells, int* rounds, int* newk) {
+int poyhip_batch_stats(poyhip_batch* b, int64_t* cells, int64_t* slots, int* rounds, int* newk) {
   if (!b) return POYHIP_EINVAL;
   poyhip_ctx* c = b->ctx;
   (void)hipSetDevice(c->device);
@@ -938,6 +938,7 @@ int poyhip_batch_stats(poyhip_batch* b, int64_t* cells, int* rounds, int* newk) 
   HIPCHK(c, hipStreamSynchronize(c->stream));
   for (int k = 0; k < b->n; k++) {
     if (cells) cells[k] = r[k].cells;
+    if (slots) slots[k] = r[k].slots;
     if (rounds) rounds[k] = r[k].rounds;
     if (newk) newk[k] = r[k].newk;
   }

@@ -78,6 +78,7 @@ struct PairResult {
   int full;       // linear: 1 when the full plane was filled
   int pad;
   long long cells;  // cells evaluated over all rounds
+  long long slots;  // lane-steps issued by the fill (cells + idle lanes)
 };
 
 // Wavefront shift right by one lane; lane 0 receives `lane0`.
