@@ -26,6 +26,7 @@ struct AffFillArgs {
   long long* stab;    // stripe tables
   PairResult* res;
   int* counter;
+  const uint8_t* cls;  // per-pair class (1: gap-free fast path), or null
 };
 
 // Reads `p[j]` for j in [0, n); other j read p[0] (the value is unused).
@@ -33,7 +34,146 @@ __device__ __forceinline__ int ldc(const uint8_t* p, int j, int n) {
   return p[(unsigned)j < (unsigned)n ? j : 0];
 }
 
-template <bool kDm>
+// Constants of one DP row (one position of the shorter sequence).
+struct AffRow {
+  int gap;       // row symbol carries the gap bit
+  int isng;      // row symbol has no gap bit
+  int ip_nogap;  // previous row symbol has no gap bit
+  int ge;        // gap extension cost(ic, gap)
+  int go;        // HAS_GAP_OPENING of the row
+  int vext;      // vertical extension cost
+  int cost_row;  // (ic & 15) << 5
+};
+
+struct AffCell {
+  int V, C, H, E, G, f, gmax;
+  uint32_t code;
+};
+
+// One cell of algn_newkk_fill_a_row_aff (src/algn.c:2113-2172): the four
+// state values, ASSIGN_MINIMUM (1936), the u16 gap counts (algn_fill_gapnum,
+// 127) and the direction byte.  Branch-free: every tie set is the set of
+// sources equal to the minimum, which is what the reference's sequence of
+// comparisons leaves.
+__device__ __forceinline__ void aff_cell(const AffRow& r, int go, int sw, int j, int jc,
+                                         int jprev, bool at_left, bool at_right,
+                                         const int* s_cost, const int* s_prep, int uV, int uCB,
+                                         int uG, int dV, int dCB, int dH, int dED, int dG,
+                                         int lH, int lCB, int lG, AffCell& o) {
+  const int gr = s_prep[jc];
+  const int sj_go = has_gap_opening(j, jprev, jc, go);
+  const bool jgap = (jc & kGap) != 0;
+  const int shx = (j == 1) ? gr : (((jprev & kGap) && !jgap) ? sj_go + gr : gr);
+  // horizontal (FILL_EXTEND_HORIZONTAL)
+  const int extH = lH + shx, opnH = lCB + sj_go + gr;
+  const int H = at_left ? kHigh : imin(extH, opnH);
+  const int endH = at_left ? 0 : (extH >= opnH);
+  // vertical (FILL_EXTEND_VERTICAL)
+  const int extV = uV + r.vext, opnV = uCB + r.go + r.ge;
+  const int V = at_right ? kHigh : imin(extV, opnV);
+  const int endV = at_right ? 0 : (extV >= opnV);
+  // gap-gap block (FILL_EXTEND_BLOCK_DIAGONAL)
+  const bool flag = r.gap && jgap;
+  const int jp_ed = at_left ? jc : jprev;
+  const bool flag2 = r.ip_nogap && !(jp_ed & kGap);
+  const int dg = flag ? 0 : kHigh;
+  const int od = flag ? (flag2 ? 0 : 2 * go) : kHigh;
+  const int extE = dED + dg, opnE = dCB + od;
+  // align (FILL_CLOSE_BLOCK_DIAGONAL)
+  const int d = s_cost[r.cost_row + (jc & 15)];
+  const int fa = dCB + d;
+  const int fv = dV + d + (r.isng ? 0 : sj_go);
+  const int fh = dH + d + (jgap ? r.go : 0);
+  const int fd = dED + d + imax(sj_go, r.go);
+  const int cmin = imin(imin(fa, fv), imin(fh, fd));
+  const bool j0 = (j == 0);
+  const int E = j0 ? kHigh : imin(extE, opnE);
+  const int endB = j0 ? 0 : (extE >= opnE);
+  const int C = j0 ? kHigh : cmin;
+  const int a2v = !j0 && fv == cmin, a2h = !j0 && fh == cmin, a2d = !j0 && fd == cmin;
+  // ASSIGN_MINIMUM
+  const int f = imin(imin(H, V), imin(E, C));
+  const int dh = H == f, dv = V == f, dd = E == f, da = C == f;
+  const int heqv = dh && (H == V);
+  // algn_fill_gapnum: each set flag contributes its source, the row keeps
+  // the maximum (u16 rows: operands < 2^16, results stored mod 2^16)
+  const int pg1 = uG & 0xFFFF, pg2 = (int)((unsigned)uG >> 16);
+  const int pg1l = dG & 0xFFFF, pg2l = (int)((unsigned)dG >> 16);
+  const int g1l = lG & 0xFFFF, g2l = (int)((unsigned)lG >> 16);
+  const int al = dd | da;
+  const int g1 = imax(imax(al ? pg1l : 0, dh ? g1l + 1 : 0), dv ? pg1 : 0);
+  const int g2 = imax(imax(al ? pg2l : 0, dh ? g2l : 0), dv ? pg2 + 1 : 0);
+  o.G = (g1 & 0xFFFF) | ((g2 & 0xFFFF) << 16);
+  o.gmax = imax(g1 & 0xFFFF, g2 & 0xFFFF);
+  const int todo = choose_dir(dd, dv, dh, sw);
+  const int amode = choose_dir(a2d, a2v, a2h, sw);
+  o.code = (uint32_t)(todo | (amode << 2) | ((endV | heqv) << 4) | ((endH | heqv) << 5) |
+                      (endB << 6));
+  o.V = V;
+  o.C = C;
+  o.H = H;
+  o.E = E;
+  o.f = f;
+}
+
+// aff_cell for a pair whose symbols past the leading gap carry no gap bit:
+// then HAS_GAP_OPENING is gap_open for every row and column past 0, the
+// horizontal/vertical extensions are the plain gap costs, the gap-gap block
+// state stays >= HIGH_NUM so it never reaches a minimum or a tie, and its
+// END_BLOCK flag is set on every column past 0.  What remains is Gotoh's
+// recurrence with POY's tie flags.
+__device__ __forceinline__ void aff_cell_fast(const AffRow& r, int go, int sw, int j, int jc,
+                                              bool at_left, bool at_right, const int* s_cost,
+                                              const int* s_prep, int uV, int uCB, int uG, int dV,
+                                              int dCB, int dH, int dG, int lH, int lCB, int lG,
+                                              AffCell& o) {
+  const int gr = s_prep[jc];
+  const int hb = lCB + go;
+  const int H = at_left ? kHigh : gr + imin(lH, hb);
+  const int endH = at_left ? 0 : (lH >= hb);
+  const int vb = uCB + go;
+  const int V = at_right ? kHigh : r.ge + imin(uV, vb);
+  const int endV = at_right ? 0 : (uV >= vb);
+  const int d = s_cost[r.cost_row + (jc & 15)];
+  const int m3 = imin(imin(dCB, dV), dH);
+  const bool j0 = (j == 0);
+  const int C = j0 ? kHigh : m3 + d;
+  const int a2v = !j0 && dV == m3, a2h = !j0 && dH == m3;
+  const int f = imin(imin(H, V), C);
+  const int dh = H == f, dv = V == f, da = C == f;
+  const int heqv = dh && (H == V);
+  const int pg1 = uG & 0xFFFF, pg2 = (int)((unsigned)uG >> 16);
+  const int pg1l = dG & 0xFFFF, pg2l = (int)((unsigned)dG >> 16);
+  const int g1l = lG & 0xFFFF, g2l = (int)((unsigned)lG >> 16);
+  const int g1 = imax(imax(da ? pg1l : 0, dh ? g1l + 1 : 0), dv ? pg1 : 0);
+  const int g2 = imax(imax(da ? pg2l : 0, dh ? g2l : 0), dv ? pg2 + 1 : 0);
+  o.G = (g1 & 0xFFFF) | ((g2 & 0xFFFF) << 16);
+  o.gmax = imax(g1 & 0xFFFF, g2 & 0xFFFF);
+  const int todo = choose_dir(0, dv, dh, sw);
+  const int amode = choose_dir(0, a2v, a2h, sw);
+  o.code = (uint32_t)(todo | (amode << 2) | ((endV | heqv) << 4) | ((endH | heqv) << 5) |
+                      ((j0 ? 0 : 1) << 6));
+  o.V = V;
+  o.C = C;
+  o.H = H;
+  o.E = kHigh;
+  o.f = f;
+}
+
+// 1 when a pair can take aff_cell_fast: no symbol past the leading gap of
+// either sequence carries the gap bit.
+__global__ void k_affine_classify(PairTable pt, uint8_t* cls) {
+  const int pid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pid >= pt.npairs) return;
+  const uint8_t* sa = pt.blob + rows_off(pt, pid);
+  const uint8_t* sb = pt.blob + cols_off(pt, pid);
+  int any = 0;
+  for (int k = 1; k < pt.len_a[pid]; k++) any |= sa[k];
+  for (int k = 1; k < pt.len_b[pid]; k++) any |= sb[k];
+  cls[pid] = (any & kGap) ? 0 : 1;
+}
+
+template <bool kDm, bool kFast>
 __global__ __launch_bounds__(64) void k_affine_fill(AffFillArgs A) {
   __shared__ int s_cost[1024];
   __shared__ int s_prep[32];
@@ -48,6 +188,7 @@ __global__ __launch_bounds__(64) void k_affine_fill(AffFillArgs A) {
     const int q = wave_grab(A.counter);
     if (q >= A.pt.n) break;
     const int pid = A.pt.order[q];
+    if (A.cls != nullptr && A.cls[pid] != (kFast ? 1 : 0)) continue;
     const uint8_t* __restrict__ sa = A.pt.blob + rows_off(A.pt, pid);
     const uint8_t* __restrict__ sb = A.pt.blob + cols_off(A.pt, pid);
     const int La = A.pt.len_a[pid], Lb = A.pt.len_b[pid];
@@ -57,7 +198,7 @@ __global__ __launch_bounds__(64) void k_affine_fill(AffFillArgs A) {
     int* ws = A.ws + (long long)blockIdx.x * A.ws_stride;
     uint8_t* dm = A.dm + (kDm ? A.pt.dm_off[pid] : 0);
     int* EDA = ws;
-    int* bnd = ws + W1;
+    int* bnd = ws + aff_bnd_offset(Lb);
     long long* stab = A.stab + A.pt.stab_off[pid];
 
     if (leni == 0) {
@@ -110,177 +251,105 @@ __global__ __launch_bounds__(64) void k_affine_fill(AffFillArgs A) {
         const int jlo = imax(i0 - newk, 0);
         const int jhi = (ilast + D + newk <= lenj - 1) ? ilast + D + newk : lenj;
         const int nsteps = (jhi - jlo + 1) + (ilast - i0);
+        const int nsteps4 = (nsteps + 3) & ~3;
         if (kDm && lane == 0) stab[b] = dmpos;
         slots += (long long)nsteps * 64;
 
-        int ic = 0, ip = 0, si_ge = 0, si_go = 0, si_ng = 0, si_vext = 0;
-        int my_lo = 1 << 30, my_hi = -1, my_rb = -1;
-        if (act) {
-          ic = sa[i];
-          ip = sa[i - 1];
-          si_ge = s_cost[(ic << 5) + kGap];
-          si_go = has_gap_opening(i, ip, ic, go);
-          si_ng = ic & 15;
-          si_vext = (i > 1 && (ip & kGap) && !(ic & kGap)) ? si_go + si_ge : si_ge;
-          my_lo = imax(i - newk, 0);
-          my_rb = i + D + newk;
-          my_hi = (my_rb <= lenj - 1) ? my_rb : lenj;
-          cells += my_hi - my_lo + 1;
-        }
-        const int* rb = bnd + ((b & 1) ^ 1) * 5 * W1;
-        int* wb = bnd + (b & 1) * 5 * W1;
-        const bool is_bnd_row = (lane == 63);
+        // row constants of lane t (row i); rows past leni get the last row's
+        // values and an empty band
+        const int ir = imin(i, leni);
+        const int ic = sa[ir], ip = sa[ir - 1];
+        AffRow row;
+        row.gap = ic & kGap;
+        row.isng = (ic & kGap) == 0;
+        row.ip_nogap = (ip & kGap) == 0;
+        row.ge = s_cost[(ic << 5) + kGap];
+        row.go = has_gap_opening(ir, ip, ic, go);
+        row.vext = (ir > 1 && (ip & kGap) && !(ic & kGap)) ? row.go + row.ge : row.ge;
+        row.cost_row = (ic & 15) << 5;
+        const int my_lo = act ? imax(i - newk, 0) : (1 << 30);
+        const int my_rb = i + D + newk;
+        const int my_hi = act ? ((my_rb <= lenj - 1) ? my_rb : lenj) : -1;
+        if (act) cells += my_hi - my_lo + 1;
+        const int4* rb4 = reinterpret_cast<const int4*>(bnd + ((b & 1) ^ 1) * 8 * W1);
+        int* wb = bnd + (b & 1) * 8 * W1;
 
-        // Up-left cell of the first step; lane 0 reads the row above the
-        // stripe, other lanes receive theirs through the pipeline.
+        // Up-left cell of lane 0's first step: row i0-1 at column jlo-1.
+        // Lane-0 sources are functions of lane 0's column jlo+s only, so the
+        // whole wavefront evaluates them with uniform addresses.
         int dV = kHigh, dCB = kHigh, dH = kHigh, dED = kHigh, dG = 0;
-        if (lane == 0 && jlo >= 1) {
+        if (jlo >= 1) {
           if (b == 0) {
-            dV = kHigh; dCB = 0; dH = HA0; dED = EDA[0]; dG = 0;
+            dCB = 0; dH = HA0; dED = EDA[0];
           } else {
-            dV = rb[jlo - 1]; dCB = rb[W1 + jlo - 1]; dH = rb[2 * W1 + jlo - 1];
-            dED = rb[3 * W1 + jlo - 1]; dG = rb[4 * W1 + jlo - 1];
+            const int4 v = rb4[2 * (jlo - 1)];
+            dV = v.x; dCB = v.y; dH = v.z; dED = v.w;
+            dG = reinterpret_cast<const int*>(rb4)[8 * (jlo - 1) + 4];
           }
         }
         int lH = kHigh, lCB = kHigh, lG = 0;
         int oV = kHigh, oCB = kHigh, oH = kHigh, oED = kHigh, oG = 0;
-        int cb0run = 0;
-        int jprev = ldc(sb, jlo - lane - 1, Lb);
-        int jc_next = ldc(sb, jlo - lane, Lb);
-        // lane 0 prefetch of the row above for the first column
-        int nV = kHigh, nCB = kHigh, nH = kHigh, nED = kHigh, nG = 0;
-        if (lane == 0 && b > 0 && jlo <= lenj) {
-          nV = rb[jlo]; nCB = rb[W1 + jlo]; nH = rb[2 * W1 + jlo];
-          nED = rb[3 * W1 + jlo]; nG = rb[4 * W1 + jlo];
-        } else if (lane == 0 && b == 0 && jlo <= lenj) {
-          nED = EDA[jlo];
-        }
-        uint32_t pack = 0;
-        for (int s = 0; s < nsteps; s++) {
-          const int j = jlo + s - lane;
-          const int jc = jc_next;
-          jc_next = ldc(sb, j + 1, Lb);
-          // values of row i0-1 at column j for lane 0
-          int sV = nV, sCB = nCB, sH = nH, sED = nED, sG = nG;
-          if (lane == 0) {
+        int cb0run = 0;                               // row-0 prefix at lane 0's column
+        int c0prev = sb[imax(jlo - 1, 0)];            // lane 0's previous column symbol
+        int jprev = sb[imax(jlo - lane - 1, 0)];      // own previous column symbol
+        for (int s = 0; s < nsteps4; s += 4) {
+          uint32_t pack = 0;
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            const int jl0 = jlo + s + u;              // lane 0's column (uniform)
+            const int jcl = imin(jl0, lenj);
+            const int c0 = sb[jcl];
+            int sV = kHigh, sCB, sH, sED, sG;
             if (b == 0) {
-              int cb0;
-              if (j == 0) {
-                cb0 = 0;
-              } else {
-                const int gr = s_prep[jc];
-                const int gop = has_gap_opening(j, jprev, jc, go);
-                cb0 = cb0run + (((jprev & kGap) && !(jc & kGap)) ? gop + gr : gr);
+              // POY's row 0 (algn_newkk_test_aff, src/algn.c:2228-2247)
+              const int gr0 = s_prep[c0];
+              const int gop0 = has_gap_opening(jl0, c0prev, c0, go);
+              const int add0 = ((c0prev & kGap) && !(c0 & kGap)) ? gop0 + gr0 : gr0;
+              cb0run = (jl0 == 0) ? 0 : cb0run + add0;
+              sCB = cb0run;
+              sH = (jl0 == 0) ? HA0 : cb0run;
+              sED = kFast ? kHigh : EDA[jcl];
+              sG = jl0 & 0xFFFF;
+            } else {
+              const int4 v = rb4[2 * jcl];
+              sV = v.x; sCB = v.y; sH = v.z; sED = v.w;
+              sG = reinterpret_cast<const int*>(rb4)[8 * jcl + 4];
+            }
+            c0prev = c0;
+            const int jc = wave_shr1(jprev, c0);
+            const int uV = wave_shr1(oV, sV);
+            const int uCB = wave_shr1(oCB, sCB);
+            const int uH = wave_shr1(oH, sH);
+            const int uED = kFast ? kHigh : wave_shr1(oED, sED);
+            const int uG = wave_shr1(oG, sG);
+            const int j = jl0 - lane;
+            AffCell o;
+            if constexpr (kFast)
+              aff_cell_fast(row, go, sw, j, jc, j == my_lo, j == my_rb, s_cost, s_prep, uV, uCB, uG,
+                            dV, dCB, dH, dG, lH, lCB, lG, o);
+            else
+              aff_cell(row, go, sw, j, jc, jprev, j == my_lo, j == my_rb, s_cost, s_prep, uV, uCB,
+                       uG, dV, dCB, dH, dED, dG, lH, lCB, lG, o);
+            const bool inb = j >= my_lo && j <= my_hi;
+            if (inb) {
+              if (i == leni && j == lenj) { rc = o.f; rgn = o.gmax; }
+              // row-0 buffer persistence across rounds: the last even row of
+              // this round that covers column j leaves its ED value there.
+              if (!kFast && !(i & 1) && i + 2 > imin(leni, j + newk)) EDA[j] = o.E;
+              if (lane == 63) {
+                reinterpret_cast<int4*>(wb)[2 * j] = make_int4(o.V, o.C, o.H, o.E);
+                wb[8 * j + 4] = o.G;
               }
-              cb0run = cb0;
-              sV = kHigh;
-              sCB = cb0;
-              sH = (j == 0) ? HA0 : cb0;
-              sG = j & 0xFFFF;
-              if (j + 1 <= lenj) nED = EDA[j + 1];
-            } else if (j + 1 <= lenj) {
-              nV = rb[j + 1]; nCB = rb[W1 + j + 1]; nH = rb[2 * W1 + j + 1];
-              nED = rb[3 * W1 + j + 1]; nG = rb[4 * W1 + j + 1];
             }
+            oV = o.V; oCB = o.C; oH = o.H; oED = o.E; oG = o.G;
+            lH = o.H; lCB = o.C; lG = o.G;
+            dV = uV; dCB = uCB; dH = uH; dED = uED; dG = uG;
+            jprev = jc;
+            pack |= o.code << (8 * u);
           }
-          const int uV = wave_shr1(oV, sV);
-          const int uCB = wave_shr1(oCB, sCB);
-          const int uH = wave_shr1(oH, sH);
-          const int uED = wave_shr1(oED, sED);
-          const int uG = wave_shr1(oG, sG);
-          uint32_t code = 0;
-          if (act && j >= my_lo && j <= my_hi) {
-            const bool at_left = (j == my_lo), at_right = (j == my_rb);
-            const int jp_ed = at_left ? jc : jprev;
-            const int gr = s_prep[jc];
-            const int sj_go = has_gap_opening(j, jprev, jc, go);
-            const int shx = (j == 1) ? gr : (((jprev & kGap) && !(jc & kGap)) ? sj_go + gr : gr);
-            int H, V, E, C;
-            int endH = 0, endV = 0, endB = 0;
-            int a2v = 0, a2h = 0, a2d = 0;
-            if (!at_left) {
-              const int ext = lH + shx, opn = lCB + sj_go + gr;
-              if (ext < opn) H = ext; else { H = opn; endH = 1; }
-            } else {
-              H = kHigh;
-            }
-            if (!at_right) {
-              const int ext = uV + si_vext, opn = uCB + si_go + si_ge;
-              if (ext < opn) V = ext; else { V = opn; endV = 1; }
-            } else {
-              V = kHigh;
-            }
-            if (j > 0) {
-              const bool flag = (kGap & ic) && (kGap & jc);
-              const bool flag2 = !(kGap & ip) && !(kGap & jp_ed);
-              const int dg = flag ? 0 : kHigh;
-              const int od = flag ? (flag2 ? 0 : 2 * go) : kHigh;
-              const int ext = dED + dg, opn = dCB + od;
-              if (ext < opn) E = ext; else { E = opn; endB = 1; }
-              const int d = s_cost[(si_ng << 5) + (jc & 15)];
-              const int extra = (sj_go < si_go) ? si_go : sj_go;
-              const int fa = dCB + d;
-              const int fv = dV + d + ((ic == si_ng) ? 0 : sj_go);
-              const int fh = dH + d + ((jc == (jc & 15)) ? 0 : si_go);
-              const int fd = dED + d + extra;
-              int cb = fa, a2a = 1;
-              if (cb >= fv) { if (cb > fv) { cb = fv; a2a = 0; a2v = 1; } else a2v = 1; }
-              if (cb >= fh) { if (cb > fh) { cb = fh; a2a = 0; a2v = 0; a2h = 1; } else a2h = 1; }
-              if (cb >= fd) { if (cb > fd) { cb = fd; a2a = 0; a2v = 0; a2h = 0; a2d = 1; } else a2d = 1; }
-              (void)a2a;
-              C = cb;
-            } else {
-              C = kHigh;
-              E = kHigh;
-            }
-            // ASSIGN_MINIMUM (src/algn.c:1936-1981)
-            int f = H, dh = 1, dv = 0, dd = 0, da = 0;
-            if (f >= V) { if (f > V) { f = V; dh = 0; dv = 1; } else dv = 1; }
-            if (f >= E) { if (f > E) { f = E; dh = 0; dv = 0; dd = 1; } else dd = 1; }
-            if (f >= C) { if (f > C) { f = C; dh = 0; dv = 0; dd = 0; da = 1; } else da = 1; }
-            const int heqv = (f == H && H == V);
-            // algn_fill_gapnum (src/algn.c:127-176), u16 rows
-            const int pg1l = dG & 0xFFFF, pg2l = (int)((unsigned)dG >> 16);
-            const int pg1 = uG & 0xFFFF, pg2 = (int)((unsigned)uG >> 16);
-            const int g1l = lG & 0xFFFF, g2l = (int)((unsigned)lG >> 16);
-            const int al = dd | da;
-            int g1, g2;
-            if (al) {
-              if (dv && dh) { g1 = imax(imax(pg1l, g1l + 1), pg1); g2 = imax(imax(pg2l, g2l), pg2 + 1); }
-              else if (dv) { g1 = imax(pg1, pg1l); g2 = imax(pg2l, pg2 + 1); }
-              else if (dh) { g1 = imax(g1l + 1, pg1l); g2 = imax(pg2l, g2l); }
-              else { g1 = pg1l; g2 = pg2l; }
-            } else if (dh) {
-              if (dv) { g1 = imax(g1l + 1, pg1); g2 = imax(pg2 + 1, g2l); }
-              else { g1 = g1l + 1; g2 = g2l; }
-            } else {
-              g1 = pg1; g2 = pg2 + 1;
-            }
-            const int G = (g1 & 0xFFFF) | ((g2 & 0xFFFF) << 16);
-            const int todo = choose_dir(dd, dv, dh, sw);
-            const int amode = choose_dir(a2d, a2v, a2h, sw);
-            code = (uint32_t)(todo | (amode << 2) | ((endV | heqv) << 4) | ((endH | heqv) << 5) | (endB << 6));
-            oV = V; oCB = C; oH = H; oED = E; oG = G;
-            lH = H; lCB = C; lG = G;
-            if (i == leni && j == lenj) { rc = f; rgn = imax(g1 & 0xFFFF, g2 & 0xFFFF); }
-            // row-0 buffer persistence across rounds: the last even row of
-            // this round that covers column j leaves its ED value there.
-            if (!(i & 1) && i + 2 > imin(leni, j + newk)) EDA[j] = E;
-            if (is_bnd_row) {
-              wb[j] = V; wb[W1 + j] = C; wb[2 * W1 + j] = H; wb[3 * W1 + j] = E; wb[4 * W1 + j] = G;
-            }
-          }
-          if (kDm) {
-            pack |= code << ((s & 3) << 3);
-            if ((s & 3) == 3 || s == nsteps - 1) {
-              reinterpret_cast<uint32_t*>(dm + dmpos)[((s >> 2) << 6) + lane] = pack;
-              pack = 0;
-            }
-          }
-          dV = uV; dCB = uCB; dH = uH; dED = uED; dG = uG;
-          jprev = jc;
+          if (kDm) reinterpret_cast<uint32_t*>(dm + dmpos)[((s >> 2) << 6) + lane] = pack;
         }
-        dmpos += (long long)((nsteps + 3) >> 2) * 256;
+        dmpos += (long long)(nsteps4 >> 2) * 256;
         __threadfence_block();
       }
       const int src = (leni - 1) & 63;
@@ -567,7 +636,8 @@ __global__ __launch_bounds__(64) void k_affine_cost_full(AffCostArgs A) {
   }
 }
 
-template __global__ void k_affine_fill<true>(AffFillArgs);
-template __global__ void k_affine_fill<false>(AffFillArgs);
+template __global__ void k_affine_fill<true, false>(AffFillArgs);
+template __global__ void k_affine_fill<true, true>(AffFillArgs);
+template __global__ void k_affine_fill<false, false>(AffFillArgs);
 
 }  // namespace poyhip

@@ -424,7 +424,14 @@ int poyhip_align_affine_3(poyhip_ctx* c, const uint8_t* si, int li, const uint8_
   A.stab = static_cast<long long*>(c->stab.p);
   A.res = dres;
   A.counter = counter;
-  hipLaunchKernelGGL(k_affine_fill<true>, dim3(1), dim3(64), 0, c->stream, A);
+  A.cls = nullptr;
+  int any = 0;
+  for (int k = 1; k < li; k++) any |= si[k];
+  for (int k = 1; k < lj; k++) any |= sj[k];
+  if (any & kGap)
+    hipLaunchKernelGGL((k_affine_fill<true, false>), dim3(1), dim3(64), 0, c->stream, A);
+  else
+    hipLaunchKernelGGL((k_affine_fill<true, true>), dim3(1), dim3(64), 0, c->stream, A);
   HIPCHK(c, hipGetLastError());
   AffBtArgs B;
   B.pt = t;
@@ -542,6 +549,7 @@ struct poyhip_batch {
   int *d_la = nullptr, *d_lb = nullptr, *d_sw = nullptr, *d_dw = nullptr, *d_dwin = nullptr;
   int* d_order = nullptr;
   int* d_counters = nullptr;
+  uint8_t* d_cls = nullptr;
   PairResult* d_res = nullptr;
   uint8_t* d_dm = nullptr;
   int* d_ws = nullptr;
@@ -582,8 +590,9 @@ int64_t ws_ints(int op, int lb) {
 
 void batch_free_device(poyhip_batch* b) {
   for (void* p : {(void*)b->d_tab, (void*)b->d_res, (void*)b->d_dm, (void*)b->d_ws,
-                  (void*)b->d_stab, (void*)b->d_counters})
+                  (void*)b->d_stab, (void*)b->d_counters, (void*)b->d_cls})
     if (p) (void)hipFree(p);
+  b->d_cls = nullptr;
   b->d_tab = nullptr;
   b->d_res = nullptr;
   b->d_dm = nullptr;
@@ -624,7 +633,12 @@ int batch_enqueue(poyhip_batch* b, const uint8_t* blob, const int64_t* off1, con
   hipStream_t st = c->stream;
   const int nch = (int)b->chunks.size();
   if (timing) HIPCHK(c, hipEventRecord(b->ev[2 * nch], st));
-  hipLaunchKernelGGL(k_zero_ints, dim3((nch + 63) / 64), dim3(64), 0, st, b->d_counters, nch);
+  hipLaunchKernelGGL(k_zero_ints, dim3((2 * nch + 63) / 64), dim3(64), 0, st, b->d_counters,
+                     2 * nch);
+  if (b->op == POYHIP_OP_ALIGN_AFFINE) {
+    PairTable all = batch_table(b, blob, off1, off2, -1);
+    hipLaunchKernelGGL(k_affine_classify, dim3((b->n + 255) / 256), dim3(256), 0, st, all, b->d_cls);
+  }
   if (!op_is_affine(b->op)) {
     PairTable all = batch_table(b, blob, off1, off2, -1);
     hipLaunchKernelGGL(k_linear_deltawh, dim3((b->n + 255) / 256), dim3(256), 0, st, all, b->d_dw,
@@ -646,6 +660,7 @@ int batch_enqueue(poyhip_batch* b, const uint8_t* blob, const int64_t* off1, con
           A.counter = b->d_counters + k;
           hipLaunchKernelGGL(k_affine_cost_full, dim3(b->grid), dim3(64), 0, st, A);
         } else {
+          // gap-free pairs on the fast kernel, the others on the general one
           AffFillArgs A;
           A.pt = t;
           A.cm = c->d_cm;
@@ -654,8 +669,11 @@ int batch_enqueue(poyhip_batch* b, const uint8_t* blob, const int64_t* off1, con
           A.ws_stride = b->ws_stride;
           A.stab = b->d_stab;
           A.res = b->d_res;
-          A.counter = b->d_counters + k;
-          hipLaunchKernelGGL(k_affine_fill<true>, dim3(b->grid), dim3(64), 0, st, A);
+          A.cls = b->d_cls;
+          A.counter = b->d_counters + 2 * k;
+          hipLaunchKernelGGL((k_affine_fill<true, true>), dim3(b->grid), dim3(64), 0, st, A);
+          A.counter = b->d_counters + 2 * k + 1;
+          hipLaunchKernelGGL((k_affine_fill<true, false>), dim3(b->grid), dim3(64), 0, st, A);
         }
         break;
       }
@@ -807,8 +825,9 @@ int poyhip_batch_create(poyhip_ctx* c, int op, int n, const int* len1, const int
   b->d_order = reinterpret_cast<int*>(p); p += 4 * (size_t)n;
   if (hipMalloc(&b->d_res, sizeof(PairResult) * (size_t)n) != hipSuccess)
     return fail(POYHIP_ENOMEM, "results");
-  if (hipMalloc(&b->d_counters, sizeof(int) * (b->chunks.size() + 1)) != hipSuccess)
+  if (hipMalloc(&b->d_counters, sizeof(int) * (2 * b->chunks.size() + 2)) != hipSuccess)
     return fail(POYHIP_ENOMEM, "counters");
+  if (hipMalloc(&b->d_cls, (size_t)n + 64) != hipSuccess) return fail(POYHIP_ENOMEM, "classes");
   if (op_has_dm(op)) {
     if (hipMalloc(&b->d_dm, (size_t)b->dm_cap + 256) != hipSuccess)
       return fail(POYHIP_ENOMEM, "direction store");

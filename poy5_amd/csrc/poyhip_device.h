@@ -108,15 +108,11 @@ __device__ __forceinline__ int has_gap_opening(int idx, int prev, int curr, int 
 
 // choose_dir (src/algn.c:1594-1624): 0 vertical, 1 horizontal, 2 diagonal,
 // 3 align.
+// Flags are 0/1 ints; written with selects and bit operations only, so the
+// uniform swap flag does not turn into a branch.
 __device__ __forceinline__ int choose_dir(int has_diag, int has_vert, int has_hrzn, int sw) {
-  if (!sw) {
-    if (has_vert) return 0;
-    if (has_hrzn) return 1;
-  } else {
-    if (has_hrzn) return 1;
-    if (has_vert) return 0;
-  }
-  return has_diag ? 2 : 3;
+  const int base = has_vert ? 0 : (has_hrzn ? 1 : (has_diag ? 2 : 3));
+  return base | (has_vert & has_hrzn & sw);
 }
 
 }  // namespace poyhip

@@ -14,9 +14,11 @@ namespace poyhip {
 POYHIP_HD inline int n_stripes(int La) { return (La - 1 + 63) >> 6; }
 
 // ---- banded affine ------------------------------------------------------
-// per-wavefront workspace ints: row-0 block-diagonal buffer (Lb) and two
-// boundary rows of five arrays (V, CB, H, ED, gap counts)
-POYHIP_HD inline int64_t aff_ws_ints(int Lb) { return (int64_t)11 * Lb + 16; }
+// per-wavefront workspace ints: row-0 block-diagonal buffer (Lb, padded to
+// 16 bytes) and two boundary rows of 8-int records (V, CB, H, ED, gap
+// counts, padding) so a record moves as one 16-byte and one 4-byte access
+POYHIP_HD inline int64_t aff_bnd_offset(int Lb) { return ((int64_t)Lb + 3) & ~(int64_t)3; }
+POYHIP_HD inline int64_t aff_ws_ints(int Lb) { return aff_bnd_offset(Lb) + (int64_t)16 * Lb + 16; }
 // direction bytes: one byte per (step, lane), at most Lb+63 steps a stripe
 POYHIP_HD inline int64_t aff_dm_bytes(int La, int Lb) {
   return (int64_t)n_stripes(La) * (((int64_t)(Lb - 1) + 64 + 3) >> 2) * 256;
