@@ -166,10 +166,15 @@ int poyhip_batch_stats(poyhip_batch *b, int64_t *cells, int64_t *slots, int *rou
 /* Captures the batch's launches into a HIP graph and replays it on later
  * runs with the same device pointers. */
 int poyhip_batch_use_graph(poyhip_batch *b, int enable);
-/* Average duration (ms) of the fill kernel over the last run, measured with
- * HIP events on the context stream. */
+/* Timing of the last non-graph run with sync set, from HIP events recorded
+ * on the context stream: last_fill_ms is the summed duration of the fill
+ * stage over all chunks (for ALIGN_AFFINE a chunk's fill stage is the
+ * gap-free kernel followed by the general kernel), last_total_ms the whole
+ * run.  -1 after a graph replay. */
 float poyhip_batch_last_fill_ms(const poyhip_batch *b);
 float poyhip_batch_last_total_ms(const poyhip_batch *b);
+/* Number of chunks the batch is split into (one fill stage each). */
+int poyhip_batch_chunks(const poyhip_batch *b);
 
 /* ---- synchronisation helpers -------------------------------------------- */
 int poyhip_sync(poyhip_ctx *ctx);

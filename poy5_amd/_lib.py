@@ -78,6 +78,7 @@ SIGNATURES = {
     "poyhip_batch_use_graph": (C.c_int, [_vp, C.c_int]),
     "poyhip_batch_last_fill_ms": (C.c_float, [_vp]),
     "poyhip_batch_last_total_ms": (C.c_float, [_vp]),
+    "poyhip_batch_chunks": (C.c_int, [_vp]),
     "poyhip_sync": (C.c_int, [_vp]),
     "poyhip_stream": (_vp, [_vp]),
     "poyhip_gen_pairs": (C.c_int, [C.c_int, C.c_int, C.c_uint64, _u8p, _i64p, _ip, _i64p]),

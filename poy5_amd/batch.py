@@ -98,6 +98,9 @@ class Batch:
     def total_ms(self) -> float:
         return float(self.L.poyhip_batch_last_total_ms(self.h))
 
+    def chunks(self) -> int:
+        return int(self.L.poyhip_batch_chunks(self.h))
+
     def stats(self):
         cells = np.zeros(self.n, np.int64)
         slots = np.zeros(self.n, np.int64)

@@ -966,5 +966,6 @@ int poyhip_batch_stats(poyhip_batch* b, int64_t* cells, int64_t* slots, int* rou
 
 float poyhip_batch_last_fill_ms(const poyhip_batch* b) { return b ? b->fill_ms : -1.f; }
 float poyhip_batch_last_total_ms(const poyhip_batch* b) { return b ? b->total_ms : -1.f; }
+int poyhip_batch_chunks(const poyhip_batch* b) { return b ? (int)b->chunks.size() : POYHIP_EINVAL; }
 
 }  // extern "C"
