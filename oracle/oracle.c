@@ -560,13 +560,8 @@ static int calc_from_aligned(const uint8_t *s1, const uint8_t *s2, int len,
 }
 
 int orc_worst_2(const uint8_t *a, const uint8_t *b, int len, const orc_cm *cm) {
-    /* Align.max_cost_2 (sequence.ml): 0 when either side is all gaps */
-    int ea = 1, eb = 1;
-    for (int i = 0; i < len; i++) {
-        if (a[i] != GAP) ea = 0;
-        if (b[i] != GAP) eb = 0;
-    }
-    if (ea || eb) return 0;
+    /* algn_CAML_worst_2 (3108) -> algn_worst_2 (3092); the all-gap test of
+     * Align.max_cost_2 (sequence.ml:652-658) is OCaml-side. */
     return calc_from_aligned(a, b, len, cm, cm->worst);
 }
 

@@ -15,7 +15,7 @@ GAP = 16
 
 
 def _idx(a, b):
-    return (a << 5) + b
+    return (int(a) << 5) + int(b)
 
 
 def all_alignments(n1, n2):

@@ -1,0 +1,242 @@
+"""Pins of the oracle (CPU) against what the mathematics fixes.
+
+Every check here is independent of the oracle's recurrences:
+  * brute-force enumeration of all alignments of tiny inputs (tests/brute.py)
+    gives the optimal linear cost and, with POY's tie order (ALIGN first, then
+    DELETE before INSERT unswapped, INSERT before DELETE swapped,
+    src/algn.c:3278-3327), the exact alignment backtrace_2d must return;
+  * unit costs without ambiguity reduce to the Levenshtein distance, for the
+    full plane and for Ukkonen bands (src/algn.c:1007-1130);
+  * re-pricing a returned alignment column by column gives the reported cost;
+  * affine (tcm (1,1), gap_open 3) on inputs without gap-bit symbols is the
+    textbook affine-gap model (one opening per maximal gap run);
+  * the 3D cube's first plane (s1 = the leading gap only) is an exact 2D
+    alignment of s2 and s3 priced by the 3D table (src/algn.c:2622-2668).
+A plausible mistake in the oracle (a dropped term, a wrong sign, a swapped
+operand or tie order, an off-by-one band edge) fails one of them.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+import brute
+import oracle
+
+BASES = (1, 2, 4, 8)
+IUPAC = (3, 5, 9, 6, 10, 12, 7, 11, 13, 14, 15)
+GAPPY = (17, 18, 20, 24, 31)
+
+
+def rseq(rng, n, amb=0.0, gapbit=0.0):
+    s = [16]
+    for _ in range(n):
+        r = rng.random()
+        if r < amb:
+            s.append(int(rng.choice(IUPAC)))
+        elif r < amb + gapbit:
+            s.append(int(rng.choice(GAPPY)))
+        else:
+            s.append(int(rng.choice(BASES)))
+    return np.array(s, np.uint8)
+
+
+def ints(a):
+    return [int(x) for x in a]
+
+
+def mutate(rng, s, k):
+    b = ints(s[1:])
+    for _ in range(k):
+        pos = int(rng.integers(0, len(b) + 1))
+        op = int(rng.integers(0, 3))
+        if op == 0 and pos < len(b):
+            b[pos] = int(rng.choice(BASES))
+        elif op == 1:
+            b.insert(pos, int(rng.choice(BASES)))
+        elif pos < len(b):
+            del b[pos]
+    return np.array([16] + b, np.uint8)
+
+
+def levenshtein(a, b):
+    a, b = ints(a[1:]), ints(b[1:])
+    d = list(range(len(b) + 1))
+    for i, x in enumerate(a, 1):
+        p = d[:]
+        d[0] = i
+        for j, y in enumerate(b, 1):
+            d[j] = min(p[j] + 1, d[j - 1] + 1, p[j - 1] + (x != y))
+    return d[-1]
+
+
+@pytest.mark.parametrize("tcm", [(1, 2), (1, 1), (2, 1), (3, 2)])
+@pytest.mark.parametrize("gapbit", [0.0, 0.3])
+def test_linear_full_plane_vs_brute(tcm, gapbit):
+    rng = np.random.default_rng(100 + tcm[0] * 10 + tcm[1] + int(gapbit * 10))
+    cm = oracle.cm_dna(*tcm)
+    C = np.array(cm.cost)
+    for _ in range(60):
+        n1 = int(rng.integers(0, 6))
+        n2 = int(rng.integers(n1, 7))
+        a, b = rseq(rng, n1, 0.15, gapbit), rseq(rng, n2, 0.15, gapbit)
+        c, dm, st = oracle.nw(a, b, cm, 100)
+        for sw in (0, 1):
+            bc, (r1, r2) = brute.best_linear(ints(a), ints(b), C, sw)
+            assert c == bc
+            x1, x2 = oracle.backtrace_2d(a, b, dm, sw)
+            assert ints(x1) == ints(r1) and ints(x2) == ints(r2), (ints(a), ints(b), sw)
+
+
+def test_linear_levenshtein_full_and_band():
+    rng = np.random.default_rng(5)
+    cm = oracle.cm_dna(1, 1)
+    C = np.array(cm.cost)
+    bands = 0
+    for _ in range(120):
+        a = rseq(rng, int(rng.integers(20, 160)))
+        b = mutate(rng, a, int(rng.integers(0, 25)))
+        if len(b) < len(a):
+            a, b = b, a
+        want = levenshtein(a, b)
+        for dw in (0, 2, 60):
+            c, dm, st = oracle.nw(a, b, cm, dw)
+            bands += 0 if st.full_plane else 1
+            assert c == want
+            r1, r2 = oracle.backtrace_2d(a, b, dm, dw & 1)
+            assert sum(int(C[(int(x) << 5) + int(y)]) for x, y in zip(r1[1:], r2[1:])) == c
+            assert oracle.verify_2(r1, r2, cm) == c
+    assert bands > 100  # the band path was exercised
+
+
+def test_linear_edge_cases():
+    cm = oracle.cm_dna(1, 2)
+    s = np.array([16, 1, 2, 4, 8, 3], np.uint8)
+    c, _, _ = oracle.nw(s, s, cm, 0)
+    assert c == 0
+    c, dm, _ = oracle.nw(np.array([16], np.uint8), s, cm, 0)
+    assert c == 2 * 5  # every symbol against a gap
+    r1, r2 = oracle.backtrace_2d(np.array([16], np.uint8), s, dm, 0)
+    assert ints(r1) == [16] * 6 and ints(r2) == ints(s)
+
+
+AFF = oracle.cm_dna(1, 1, 3)
+_AC = np.array(AFF.cost)
+
+
+def _sub(a, b):
+    return int(_AC[(int(a) << 5) + int(b)])
+
+
+def _ge(x):
+    return int(_AC[(int(x) << 5) + 16])
+
+
+def test_affine_vs_textbook():
+    rng = np.random.default_rng(11)
+    for t in range(150):
+        n1 = int(rng.integers(0, 6))
+        n2 = int(rng.integers(n1, 7))
+        a, b = rseq(rng, n1, 0.2), rseq(rng, n2, 0.2)
+        want = brute.best_affine(ints(a), ints(b), _sub, 3, _ge)
+        r = oracle.align_affine_3(a, b, AFF, t & 1)
+        assert r["cost"] == want, (ints(a), ints(b))
+        assert oracle.cost_affine_3(a, b, AFF) == want
+        assert oracle.cost_affine_3(b, a, AFF) == want
+        assert brute.affine_cost_of_aligned(r["resi"], r["resj"], _sub, 3, _ge) == want
+        assert oracle.verify_2(r["resi"], r["resj"], AFF) == want
+
+
+def test_affine_band_rounds_agree_with_full():
+    """Banded doubling (align) and the full DP (cost-only) agree on inputs
+    without gap-bit symbols, at lengths where the band is narrow."""
+    rng = np.random.default_rng(12)
+    multi = 0
+    for t in range(40):
+        a = rseq(rng, int(rng.integers(50, 400)), 0.05)
+        b = mutate(rng, a, int(rng.integers(0, 40)))
+        if len(b) < len(a):
+            a, b = b, a
+        r = oracle.align_affine_3(a, b, AFF, t & 1)
+        multi += r["stats"].rounds > 1
+        assert r["cost"] == oracle.cost_affine_3(a, b, AFF)
+        assert oracle.verify_2(r["resi"], r["resj"], AFF) == r["cost"]
+    assert multi > 5
+
+
+def test_affine_medians_follow_alignment():
+    """median_with_gaps is the column-wise median table of the alignment;
+    the median drops the gap columns (inputs without gap-bit symbols)."""
+    rng = np.random.default_rng(13)
+    med = np.array(AFF.median)
+    for t in range(40):
+        a = rseq(rng, int(rng.integers(1, 60)), 0.1)
+        b = mutate(rng, a, int(rng.integers(0, 8)))
+        if len(b) < len(a):
+            a, b = b, a
+        r = oracle.align_affine_3(a, b, AFF, 0)
+        ri, rj, mw, m = ints(r["resi"]), ints(r["resj"]), ints(r["medianwg"]), ints(r["median"])
+        assert len(ri) == len(rj) == len(mw)
+        for x, y, z in zip(ri[1:], rj[1:], mw[1:]):
+            if x != 16 and y != 16:
+                assert z == int(med[(x << 5) + y])
+            else:
+                assert z == (x | y)
+        assert m[0] == 16
+
+
+def test_linear_medians_closed_forms():
+    rng = np.random.default_rng(14)
+    cm = oracle.cm_dna(1, 2)
+    med = np.array(cm.median)
+    for _ in range(40):
+        n = int(rng.integers(1, 30))
+        a = rseq(rng, n, 0.2, 0.2)
+        b = rseq(rng, n, 0.2, 0.2)
+        cols = [int(med[(int(x) << 5) + int(y)]) for x, y in zip(a, b)]
+        assert ints(oracle.median_2_with_gaps(a, b, cm)) == cols
+        kept = [v for v in cols if v != 16]
+        assert ints(oracle.median_2_no_gaps(a, b, cm)) == [16] + kept
+        anc = ints(oracle.ancestor_2(a, b, cm))
+        assert anc == (kept if kept and kept[0] == 16 else [16] + kept)
+
+
+def _brute_3d_plane0(s2, s3, c3):
+    """Min over all 2D alignments of s2 and s3 with columns priced by the
+    3D table with the first code fixed to the gap."""
+    cost = np.array(c3.cost)
+
+    def e(b, c):
+        return int(cost[(((16 << 5) + b) << 5) + c])
+
+    best = None
+    for mv in brute.all_alignments(len(s2) - 1, len(s3) - 1):
+        i = j = 1
+        tot = 0
+        for m in mv:
+            if m == "A":
+                tot += e(s2[i], s3[j]); i += 1; j += 1
+            elif m == "D":
+                tot += e(s2[i], 16); i += 1
+            else:
+                tot += e(16, s3[j]); j += 1
+        best = tot if best is None else min(best, tot)
+    return best
+
+
+def test_3d_first_plane_vs_brute():
+    rng = np.random.default_rng(15)
+    c3 = oracle.cm3_from_2d(oracle.cm_dna(1, 1))
+    for _ in range(50):
+        s1 = np.array([16], np.uint8)
+        s2 = rseq(rng, int(rng.integers(0, 5)), 0.2)
+        s3 = rseq(rng, int(rng.integers(0, 5)), 0.2)
+        cost, dm = oracle.nw_3d(s1, s2, s3, c3)
+        assert cost == _brute_3d_plane0(ints(s2), ints(s3), c3)
+        out = oracle.backtrack_3d(s1, s2, s3, dm)
+        assert out is not None
+        r1, r2, r3 = (ints(x) for x in out)
+        assert all(x == 16 for x in r1)
+        assert [x for x in r2 if x != 16] == [x for x in ints(s2) if x != 16]
+        assert [x for x in r3 if x != 16] == [x for x in ints(s3) if x != 16]
+

@@ -1,0 +1,126 @@
+"""Cost tables (CPU): the oracle's and the product's tables against closed
+forms the text of src/cost_matrix.ml fixes, and against each other.
+
+Closed forms used (src/cost_matrix.ml:1337-1348 builds the 5x5 tcm;
+862-897 fills sets for the linear model; 721-804 refills them bitwise after
+set_cost_model (Affine g), 1003-1016; 994 fills prepend/tail; 1605-1651
+builds the 3D tables):
+  * singletons: cost(x, x) = 0, cost(base, base') = trans, cost(x, gap) =
+    gaps, median(x, y) = x | y when trans < 2 * gaps;
+  * metric sets: cost(I, J) = min over a in I, b in J of cost(a, b);
+  * prepend[x] = cost(gap, x), tail[x] = cost(x, gap), delta = min positive;
+  * 3D: cost3 is symmetric in its three codes, cost3(a, a, b) = cost(a, b)
+    for singletons, and the hand-derived entries below.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+import oracle
+from poy5_amd import cost_matrix
+
+BASES = (1, 2, 4, 8)
+SINGLES = (1, 2, 4, 8, 16)
+A, C, G, T, GAP = 1, 2, 4, 8, 16
+
+
+def bits(x):
+    return [s for s in SINGLES if x & s]
+
+
+def arr(cm, name):
+    return np.array(getattr(cm, name))
+
+
+@pytest.mark.parametrize("trans,gaps", [(1, 2), (1, 1), (2, 3), (3, 2)])
+def test_linear_singletons(trans, gaps):
+    cm = oracle.cm_dna(trans, gaps)
+    cost, med = arr(cm, "cost"), arr(cm, "median")
+    for a in SINGLES:
+        for b in SINGLES:
+            k = (a << 5) + b
+            want = 0 if a == b else (gaps if GAP in (a, b) else trans)
+            assert cost[k] == want, (a, b)
+            if a == b:
+                assert med[k] == a
+            elif a != GAP and b != GAP and trans < 2 * gaps:
+                assert med[k] == a | b
+
+
+@pytest.mark.parametrize("trans,gaps", [(1, 2), (1, 1), (2, 3)])
+def test_linear_metric_sets(trans, gaps):
+    cm = oracle.cm_dna(trans, gaps)
+    cost = arr(cm, "cost")
+    for i in range(1, 32):
+        for j in range(1, 32):
+            want = min(cost[(a << 5) + b] for a in bits(i) for b in bits(j))
+            assert cost[(i << 5) + j] == want, (i, j)
+
+
+@pytest.mark.parametrize("args", [(1, 2, None), (1, 1, None), (1, 1, 3), (2, 1, 2)])
+def test_prepend_tail_delta(args):
+    cm = oracle.cm_dna(args[0], args[1], args[2])
+    cost, pre, tail = arr(cm, "cost"), arr(cm, "prepend"), arr(cm, "tail")
+    for x in range(1, 32):
+        assert pre[x] == cost[(GAP << 5) + x]
+        assert tail[x] == cost[(x << 5) + GAP]
+    pos = cost[cost > 0]
+    assert oracle.min_non0_cost(cm) == pos.min()
+
+
+def test_affine_unit_hand_entries():
+    """Bitwise refill with tcm (1,1), gap_open 3 (the affine workload)."""
+    cm = oracle.cm_dna(1, 1, 3)
+    cost, med = arr(cm, "cost"), arr(cm, "median")
+    assert cm.cost_model == 1 and cm.gap_open == 3
+    assert cost[(A << 5) + A] == 0 and med[(A << 5) + A] == A
+    assert cost[(A << 5) + C] == 1 and med[(A << 5) + C] == A | C
+    assert cost[(A << 5) + GAP] == 1 and med[(A << 5) + GAP] == A | GAP
+    # A vs the set {A,C}: the shared state costs nothing
+    assert cost[(A << 5) + (A | C)] == 0 and med[(A << 5) + (A | C)] == A
+    # {A,C} vs {G,T}: one substitution, every state pair ties
+    assert cost[((A | C) << 5) + (G | T)] == 1
+    assert med[((A | C) << 5) + (G | T)] == A | C | G | T
+
+
+@pytest.mark.parametrize("args", [(1, 2, None), (1, 1, None), (1, 1, 3), (2, 3, None), (2, 1, 2)])
+def test_product_tables_match_oracle(args):
+    """poy5_amd.cost_matrix (Python) and the oracle (C) are separate
+    restatements of the same OCaml table builders."""
+    o = oracle.cm_dna(args[0], args[1], args[2])
+    p = cost_matrix.of_transformations_and_gaps(*args)
+    for name in ("cost", "median", "worst", "prepend", "tail"):
+        po = np.asarray(getattr(p, name)).astype(np.int64)
+        oo = arr(o, name).astype(np.int64)
+        n = min(len(po), len(oo))
+        assert np.array_equal(po[:n], oo[:n]), name
+    assert p.cost_model == o.cost_model
+    assert p.min_non0_cost() == oracle.min_non0_cost(o)
+
+
+def test_3d_tables():
+    c2 = oracle.cm_dna(1, 1)
+    c3 = oracle.cm3_from_2d(c2)
+    cost, med = np.array(c3.cost), np.array(c3.median)
+
+    def e(a, b, c):
+        k = (((a << 5) + b) << 5) + c
+        return int(cost[k]), int(med[k])
+
+    # hand-derived (uniform 1/1): the Steiner state of three leaves
+    assert e(A, A, A) == (0, A)
+    assert e(A, A, C) == (1, A)
+    assert e(A, C, G)[0] == 2
+    assert e(A, GAP, GAP) == (1, GAP)
+    assert e(GAP, GAP, GAP)[0] == 0
+    c2c = np.array(c2.cost)
+    for a, b, c in itertools.product(range(1, 32), repeat=3):
+        if (a * 7 + b * 3 + c) % 5:  # a deterministic subset keeps this quick
+            continue
+        v = e(a, b, c)[0]
+        for p in set(itertools.permutations((a, b, c))):
+            assert e(*p)[0] == v
+    for a in SINGLES:
+        for b in SINGLES:
+            assert e(a, a, b)[0] == c2c[(a << 5) + b]
