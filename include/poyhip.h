@@ -120,6 +120,40 @@ int poyhip_median_2_with_gaps(poyhip_ctx *ctx, const uint8_t *a, const uint8_t *
 int poyhip_worst_2(poyhip_ctx *ctx, const uint8_t *a, const uint8_t *b, int len,
                    int *cost);
 
+/* ---- three sequences ----------------------------------------------------- */
+/* Cost and median entries of POY's struct cm_3d (src/cm.h:253-280) for 5-bit
+ * codes, index ((a << 5) + b) << 5 + c, as Cost_matrix.Three_D pushes them
+ * through cm_CAML_create_3d / cm_CAML_set_cost_3d / cm_CAML_set_median_3d
+ * (src/cm.c:2433, 2333, 2369). */
+typedef struct {
+  int cost[32768];
+  uint8_t median[32768];
+} poyhip_cm3;
+int poyhip_set_cm3(poyhip_ctx *ctx, const poyhip_cm3 *cm3);
+
+/* algn_CAML_simple_3 (src/algn.c:3194) = Align.cost_3: algn_nw_3d's cube
+ * cost (algn_fill_cube, src/algn.c:2593, with its row-pointer arithmetic).
+ * `uk` is accepted and unused, as algn_fill_cube ignores it.  Leaves the
+ * directions in the context for poyhip_backtrack_3d. */
+int poyhip_cost_3(poyhip_ctx *ctx, const uint8_t *s1, int l1, const uint8_t *s2, int l2,
+                  const uint8_t *s3, int l3, int uk, int *cost);
+/* algn_CAML_backtrack_3d (src/algn.c:3477) = Align.extract_edited_3: the
+ * three aligned sequences of the last poyhip_cost_3 on the same inputs
+ * (the walk stops at the origin cell, as backtrack_3d does);
+ * cap >= l1 + l2 + l3. */
+int poyhip_backtrack_3d(poyhip_ctx *ctx, const uint8_t *s1, int l1, const uint8_t *s2,
+                        int l2, const uint8_t *s3, int l3, uint8_t *r1, uint8_t *r2,
+                        uint8_t *r3, int cap, int *len);
+/* algn_CAML_align_3d (src/algn.c:3521) = Align.c_align_3: fill + backtrack. */
+int poyhip_align_3d(poyhip_ctx *ctx, const uint8_t *s1, int l1, const uint8_t *s2, int l2,
+                    const uint8_t *s3, int l3, int uk, uint8_t *r1, uint8_t *r2,
+                    uint8_t *r3, int cap, int *len, int *cost);
+/* algn_CAML_median_3 (src/algn.c:3680) = Align.c_median_3: like
+ * algn_get_median_3d, every position receives the median of the last
+ * column; out_len = len. */
+int poyhip_median_3(poyhip_ctx *ctx, const uint8_t *a, const uint8_t *b, const uint8_t *c,
+                    int len, uint8_t *out, int cap, int *out_len);
+
 /* ---- batches (no reference counterpart) --------------------------------- */
 /* Operation of a batch. */
 #define POYHIP_OP_ALIGN_AFFINE 1 /* align_affine_3 per pair (shorter first  */
@@ -176,6 +210,28 @@ float poyhip_batch_last_total_ms(const poyhip_batch *b);
 /* Number of chunks the batch is split into (one fill stage each). */
 int poyhip_batch_chunks(const poyhip_batch *b);
 
+/* Batches of triples (no reference counterpart). */
+#define POYHIP_OP_ALIGN_3D 5 /* algn_nw_3d + backtrack_3d + median_3 per triple */
+#define POYHIP_OP_COST_3D 6  /* algn_nw_3d cost per triple                      */
+typedef struct poyhip_batch3 poyhip_batch3;
+/* Plans n triples (lengths with the leading gap); chunks keep each chunk's
+ * direction stores within dm_budget_bytes (0 = automatic). */
+int poyhip_batch3_create(poyhip_ctx *ctx, int op, int n, const int *len1, const int *len2,
+                         const int *len3, size_t dm_budget_bytes, poyhip_batch3 **out);
+int poyhip_batch3_destroy(poyhip_batch3 *b);
+/* ALIGN_3D writes four regions of cap_k = l1+l2+l3 bytes per triple (r1,
+ * r2, r3, median_3), each right-aligned; d_outlen[4k..4k+3] = lengths
+ * (-1 when the walk left the cube). */
+size_t poyhip_batch3_out_bytes(const poyhip_batch3 *b);
+int64_t poyhip_batch3_out_offset(const poyhip_batch3 *b, int k);
+int poyhip_batch3_out_cap(const poyhip_batch3 *b, int k);
+int poyhip_batch3_run(poyhip_batch3 *b, const uint8_t *d_blob, const int64_t *d_off1,
+                      const int64_t *d_off2, const int64_t *d_off3, int *d_cost,
+                      uint8_t *d_out, int *d_outlen, int sync);
+int poyhip_batch3_chunks(const poyhip_batch3 *b);
+/* Summed fill duration of the last run with sync set (HIP events). */
+float poyhip_batch3_last_fill_ms(const poyhip_batch3 *b);
+
 /* ---- synchronisation helpers -------------------------------------------- */
 int poyhip_sync(poyhip_ctx *ctx);
 /* Opaque hipStream_t of the context (for callers that enqueue their own
@@ -189,6 +245,10 @@ void *poyhip_stream(poyhip_ctx *ctx);
  * arrays have 2n entries (first, second sequence of each pair). */
 int poyhip_gen_pairs(int config, int n, uint64_t seed, uint8_t *blob, int64_t *off,
                      int *len, int64_t *total_bytes);
+/* Triples of configuration 5 (three mutations of one root); off/len arrays
+ * have 3n entries. */
+int poyhip_gen_triples(int config, int n, uint64_t seed, uint8_t *blob, int64_t *off,
+                       int *len, int64_t *total_bytes);
 
 #ifdef __cplusplus
 }

@@ -23,6 +23,8 @@ OP_ALIGN_AFFINE = 1
 OP_COST_AFFINE = 2
 OP_ALIGN_2D = 3
 OP_COST_2D = 4
+OP_ALIGN_3D = 5
+OP_COST_3D = 6
 
 
 class PoyhipCM(C.Structure):
@@ -33,6 +35,10 @@ class PoyhipCM(C.Structure):
         ("prepend", C.c_int * 64), ("tail", C.c_int * 64),
         ("median", C.c_uint8 * 2048),
     ]
+
+
+class PoyhipCM3(C.Structure):
+    _fields_ = [("cost", C.c_int * 32768), ("median", C.c_uint8 * 32768)]
 
 
 class PoyhipError(RuntimeError):
@@ -67,6 +73,23 @@ SIGNATURES = {
     "poyhip_median_2": (C.c_int, [_vp, _u8p, _u8p, C.c_int, _u8p, C.c_int, _ip]),
     "poyhip_median_2_with_gaps": (C.c_int, [_vp, _u8p, _u8p, C.c_int, _u8p, C.c_int, _ip]),
     "poyhip_worst_2": (C.c_int, [_vp, _u8p, _u8p, C.c_int, _ip]),
+    "poyhip_set_cm3": (C.c_int, [_vp, C.POINTER(PoyhipCM3)]),
+    "poyhip_cost_3": (C.c_int, [_vp, _u8p, C.c_int, _u8p, C.c_int, _u8p, C.c_int, C.c_int, _ip]),
+    "poyhip_backtrack_3d": (C.c_int, [_vp, _u8p, C.c_int, _u8p, C.c_int, _u8p, C.c_int, _u8p,
+                                      _u8p, _u8p, C.c_int, _ip]),
+    "poyhip_align_3d": (C.c_int, [_vp, _u8p, C.c_int, _u8p, C.c_int, _u8p, C.c_int, C.c_int,
+                                  _u8p, _u8p, _u8p, C.c_int, _ip, _ip]),
+    "poyhip_median_3": (C.c_int, [_vp, _u8p, _u8p, _u8p, C.c_int, _u8p, C.c_int, _ip]),
+    "poyhip_batch3_create": (C.c_int, [_vp, C.c_int, C.c_int, _ip, _ip, _ip, C.c_size_t,
+                                       C.POINTER(_vp)]),
+    "poyhip_batch3_destroy": (C.c_int, [_vp]),
+    "poyhip_batch3_out_bytes": (C.c_size_t, [_vp]),
+    "poyhip_batch3_out_offset": (C.c_int64, [_vp, C.c_int]),
+    "poyhip_batch3_out_cap": (C.c_int, [_vp, C.c_int]),
+    "poyhip_batch3_run": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_int]),
+    "poyhip_batch3_chunks": (C.c_int, [_vp]),
+    "poyhip_batch3_last_fill_ms": (C.c_float, [_vp]),
+    "poyhip_gen_triples": (C.c_int, [C.c_int, C.c_int, C.c_uint64, _u8p, _i64p, _ip, _i64p]),
     "poyhip_batch_create": (C.c_int, [_vp, C.c_int, C.c_int, _ip, _ip, _ip, C.c_size_t,
                                       C.POINTER(_vp)]),
     "poyhip_batch_destroy": (C.c_int, [_vp]),

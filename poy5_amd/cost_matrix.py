@@ -156,3 +156,57 @@ def of_transformations_and_gaps(trans: int, gaps: int, gap_open: int | None = No
 def default_nucleotides() -> TwoD:
     """``Cost_matrix.Two_D.default`` (cost_matrix.ml:1297): unit costs."""
     return of_transformations_and_gaps(1, 1)
+
+
+@dataclass
+class ThreeD:
+    """Tables of ``Cost_matrix.Three_D`` for 5-bit codes: entry (a, b, c) at
+    ``((a << 5) + b) << 5 + c``."""
+    cost: np.ndarray      # int32[32768]
+    median: np.ndarray    # uint8[32768]
+
+    def to_c(self) -> _lib.PoyhipCM3:
+        s = _lib.PoyhipCM3()
+        s.cost[:] = [int(x) for x in self.cost]
+        s.median[:] = [int(x) for x in self.median]
+        return s
+
+
+def is_metric(m: TwoD) -> bool:
+    """Triangle inequality over the five single states."""
+    for a in SINGLES:
+        for b in SINGLES:
+            for c in SINGLES:
+                if m.c(a, c) > m.c(a, b) + m.c(b, c):
+                    return False
+    return True
+
+
+def three_d_of_two_dim_comb(m: TwoD) -> ThreeD:
+    """``Three_D.of_two_dim_comb`` (cost_matrix.ml:1605-1651): for every
+    triple of codes, the cheapest single state l (a "Steiner" state of the
+    three) prices c(l, i) + c(l, j) + c(l, k); the median is the lowest tied
+    state (pick_bit).  For a non-metric 2D model the gap state is only
+    allowed when at least two of the three codes contain it."""
+    metric = is_metric(m)
+    cost = np.zeros(32768, dtype=np.int32)
+    median = np.zeros(32768, dtype=np.uint8)
+    big = 2 ** 62
+    for i in range(1, 32):
+        for j in range(1, 32):
+            for k in range(1, 32):
+                best, med = big, 0
+                for inter in SINGLES:
+                    shared = (i & inter != 0) + (j & inter != 0) + (k & inter != 0)
+                    if metric or shared >= 2 or inter != GAP:
+                        c = m.c(inter, i) + m.c(inter, j) + m.c(inter, k)
+                    else:
+                        c = big
+                    if c < best:
+                        best, med = c, inter
+                    elif c == best:
+                        med |= inter
+                idx = (((i << LCM) + j) << LCM) + k
+                cost[idx] = best if best < big else np.iinfo(np.int32).max
+                median[idx] = med & -med
+    return ThreeD(cost, median)

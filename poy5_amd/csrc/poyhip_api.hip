@@ -18,6 +18,7 @@
 #include "poyhip_layout.h"
 
 #include "affine.hip"
+#include "cube.hip"
 #include "linear.hip"
 #include "medians.hip"
 
@@ -58,6 +59,11 @@ struct poyhip_ctx {
   bool last2d = false;
   int last_deltawh = 0;
   std::vector<uint8_t> last_s1, last_s2;
+  // three-sequence model and the state of the last poyhip_cost_3
+  Cm3Dev* d_cm3 = nullptr;
+  bool has_cm3 = false;
+  bool last3d = false;
+  std::vector<uint8_t> last3[3];
 };
 
 namespace {
@@ -244,6 +250,7 @@ int poyhip_flush(poyhip_ctx* c) {
     b->cap = 0;
   }
   c->last2d = false;
+  c->last3d = false;
   return POYHIP_OK;
 }
 
@@ -251,6 +258,7 @@ int poyhip_destroy(poyhip_ctx* c) {
   if (!c) return POYHIP_EINVAL;
   poyhip_flush(c);
   if (c->d_cm) (void)hipFree(c->d_cm);
+  if (c->d_cm3) (void)hipFree(c->d_cm3);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return POYHIP_OK;
@@ -967,5 +975,471 @@ int poyhip_batch_stats(poyhip_batch* b, int64_t* cells, int64_t* slots, int* rou
 float poyhip_batch_last_fill_ms(const poyhip_batch* b) { return b ? b->fill_ms : -1.f; }
 float poyhip_batch_last_total_ms(const poyhip_batch* b) { return b ? b->total_ms : -1.f; }
 int poyhip_batch_chunks(const poyhip_batch* b) { return b ? (int)b->chunks.size() : POYHIP_EINVAL; }
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Three sequences (algn_nw_3d, backtrack_3d, algn_get_median_3d)
+// ---------------------------------------------------------------------------
+
+namespace {
+
+struct OneDesc3 {
+  int64_t off1, off2, off3, dm_off, out_off;
+  int l1, l2, l3, order, counter, pad;
+};
+static_assert(sizeof(OneDesc3) <= kDescBytes, "triple descriptor block");
+
+TripleTable triple_table_one(void* dstage) {
+  char* base = static_cast<char*>(dstage);
+  OneDesc3* d = reinterpret_cast<OneDesc3*>(base);
+  TripleTable t;
+  t.blob = reinterpret_cast<const uint8_t*>(base + kDescBytes);
+  t.off1 = &d->off1;
+  t.off2 = &d->off2;
+  t.off3 = &d->off3;
+  t.len1 = &d->l1;
+  t.len2 = &d->l2;
+  t.len3 = &d->l3;
+  t.dm_off = &d->dm_off;
+  t.out_off = &d->out_off;
+  t.order = &d->order;
+  t.n = 1;
+  t.ntriples = 1;
+  return t;
+}
+
+// One triple on the context: fill (directions kept in c->dm) and, when
+// asked, the backtrack.  outblk = [cost, 4 lengths | outputs].
+int run_cube_one(poyhip_ctx* c, const uint8_t* s1, int l1, const uint8_t* s2, int l2,
+                 const uint8_t* s3, int l3, bool backtrack, int* cost, int outlen[4],
+                 std::vector<uint8_t>* outs) {
+  const size_t need = kDescBytes + (size_t)l1 + l2 + l3;
+  int rc = grow(c, c->stage, need);
+  if (rc) return rc;
+  const int64_t dmb = cube_dm_bytes(l1, l2, l3);
+  const int64_t outb = cube_out_bytes(l1, l2, l3);
+  if ((rc = grow(c, c->dm, dmb + 256))) return rc;
+  if ((rc = grow(c, c->ws, cube_ring_ints(l1, l2, l3) * sizeof(int)))) return rc;
+  if ((rc = grow(c, c->outblk, kResBytes + outb))) return rc;
+  c->h_stage.resize(need);
+  OneDesc3 d;
+  memset(&d, 0, sizeof d);
+  d.off1 = 0;
+  d.off2 = l1;
+  d.off3 = (int64_t)l1 + l2;
+  d.l1 = l1;
+  d.l2 = l2;
+  d.l3 = l3;
+  memcpy(c->h_stage.data(), &d, sizeof d);
+  memcpy(c->h_stage.data() + kDescBytes, s1, l1);
+  memcpy(c->h_stage.data() + kDescBytes + l1, s2, l2);
+  memcpy(c->h_stage.data() + kDescBytes + l1 + l2, s3, l3);
+  HIPCHK(c, hipMemcpyAsync(c->stage.p, c->h_stage.data(), need, hipMemcpyHostToDevice, c->stream));
+  TripleTable t = triple_table_one(c->stage.p);
+  int* dcost = static_cast<int*>(c->outblk.p);
+  int* doutlen = dcost + 4;
+  uint8_t* dout = static_cast<uint8_t*>(c->outblk.p) + kResBytes;
+  CubeFillArgs A;
+  A.tt = t;
+  A.cm3 = c->d_cm3;
+  A.dm = static_cast<uint8_t*>(c->dm.p);
+  A.ring = static_cast<int*>(c->ws.p);
+  A.ring_stride = 0;
+  A.cost = dcost;
+  A.counter = &reinterpret_cast<OneDesc3*>(c->stage.p)->counter;
+  hipLaunchKernelGGL(k_cube_fill<true>, dim3(1), dim3(64), 0, c->stream, A);
+  HIPCHK(c, hipGetLastError());
+  size_t back = 4 * sizeof(int);
+  if (backtrack) {
+    CubeBtArgs B;
+    B.tt = t;
+    B.cm3 = c->d_cm3;
+    B.dm = A.dm;
+    B.out = dout;
+    B.outlen = doutlen;
+    hipLaunchKernelGGL(k_cube_backtrack, dim3(1), dim3(64), 0, c->stream, B);
+    HIPCHK(c, hipGetLastError());
+    back = kResBytes + outb;
+  }
+  c->h_out.resize(back);
+  HIPCHK(c, hipMemcpyAsync(c->h_out.data(), c->outblk.p, back, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  memcpy(cost, c->h_out.data(), sizeof(int));
+  if (backtrack) {
+    memcpy(outlen, c->h_out.data() + 4 * sizeof(int), 4 * sizeof(int));
+    outs->assign(c->h_out.begin() + kResBytes, c->h_out.begin() + kResBytes + outb);
+  }
+  return POYHIP_OK;
+}
+
+bool valid_triple(const uint8_t* s1, int l1, const uint8_t* s2, int l2, const uint8_t* s3,
+                  int l3) {
+  return valid_seq(s1, l1) && valid_seq(s2, l2) && valid_seq(s3, l3);
+}
+
+int copy_triple_outputs(poyhip_ctx* c, int l1, int l2, int l3, const int outlen[4],
+                        const std::vector<uint8_t>& blk, uint8_t* r1, uint8_t* r2, uint8_t* r3,
+                        int cap, int* len) {
+  if (outlen[0] < 0) return set_err(c, POYHIP_EDEVICE, "3D backtrack left the cube");
+  if (outlen[0] > cap) return set_err(c, POYHIP_ECAP, "capacity %d < %d", cap, outlen[0]);
+  const int cp = (int)cube_out_cap(l1, l2, l3);
+  const int n = outlen[0];
+  memcpy(r1, blk.data() + cp - n, n);
+  memcpy(r2, blk.data() + 2 * cp - n, n);
+  memcpy(r3, blk.data() + 3 * cp - n, n);
+  *len = n;
+  return POYHIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int poyhip_set_cm3(poyhip_ctx* c, const poyhip_cm3* cm3) {
+  if (!c || !cm3) return POYHIP_EINVAL;
+  (void)hipSetDevice(c->device);
+  if (!c->d_cm3 && hipMalloc(&c->d_cm3, sizeof(Cm3Dev)) != hipSuccess) {
+    c->d_cm3 = nullptr;
+    return set_err(c, POYHIP_ENOMEM, "hipMalloc of the 3D tables failed");
+  }
+  static_assert(sizeof(Cm3Dev) == sizeof(poyhip_cm3), "3D table layout");
+  HIPCHK(c, hipMemcpyAsync(c->d_cm3, cm3, sizeof(Cm3Dev), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->has_cm3 = true;
+  c->last3d = false;
+  return POYHIP_OK;
+}
+
+int poyhip_cost_3(poyhip_ctx* c, const uint8_t* s1, int l1, const uint8_t* s2, int l2,
+                  const uint8_t* s3, int l3, int uk, int* cost) {
+  (void)uk;  // algn_fill_cube ignores its Ukkonen barrier argument
+  if (!c || !cost || !valid_triple(s1, l1, s2, l2, s3, l3)) return POYHIP_EINVAL;
+  if (!c->has_cm3) return set_err(c, POYHIP_ESTATE, "no 3D cost matrix installed");
+  (void)hipSetDevice(c->device);
+  int outlen[4];
+  int rc = run_cube_one(c, s1, l1, s2, l2, s3, l3, false, cost, outlen, nullptr);
+  if (rc) return rc;
+  c->last3d = true;
+  c->last3[0].assign(s1, s1 + l1);
+  c->last3[1].assign(s2, s2 + l2);
+  c->last3[2].assign(s3, s3 + l3);
+  return POYHIP_OK;
+}
+
+int poyhip_backtrack_3d(poyhip_ctx* c, const uint8_t* s1, int l1, const uint8_t* s2, int l2,
+                        const uint8_t* s3, int l3, uint8_t* r1, uint8_t* r2, uint8_t* r3, int cap,
+                        int* len) {
+  if (!c || !r1 || !r2 || !r3 || !len || !valid_triple(s1, l1, s2, l2, s3, l3))
+    return POYHIP_EINVAL;
+  const uint8_t* s[3] = {s1, s2, s3};
+  const int l[3] = {l1, l2, l3};
+  bool same = c->last3d;
+  for (int u = 0; u < 3 && same; u++)
+    same = (int)c->last3[u].size() == l[u] && !memcmp(c->last3[u].data(), s[u], l[u]);
+  if (!same) return set_err(c, POYHIP_ESTATE, "backtrack without a matching poyhip_cost_3");
+  (void)hipSetDevice(c->device);
+  int cost, outlen[4];
+  std::vector<uint8_t> blk;
+  int rc = run_cube_one(c, s1, l1, s2, l2, s3, l3, true, &cost, outlen, &blk);
+  if (rc) return rc;
+  return copy_triple_outputs(c, l1, l2, l3, outlen, blk, r1, r2, r3, cap, len);
+}
+
+int poyhip_align_3d(poyhip_ctx* c, const uint8_t* s1, int l1, const uint8_t* s2, int l2,
+                    const uint8_t* s3, int l3, int uk, uint8_t* r1, uint8_t* r2, uint8_t* r3,
+                    int cap, int* len, int* cost) {
+  (void)uk;
+  if (!c || !cost || !r1 || !r2 || !r3 || !len || !valid_triple(s1, l1, s2, l2, s3, l3))
+    return POYHIP_EINVAL;
+  if (!c->has_cm3) return set_err(c, POYHIP_ESTATE, "no 3D cost matrix installed");
+  (void)hipSetDevice(c->device);
+  int outlen[4];
+  std::vector<uint8_t> blk;
+  int rc = run_cube_one(c, s1, l1, s2, l2, s3, l3, true, cost, outlen, &blk);
+  if (rc) return rc;
+  c->last3d = true;
+  c->last3[0].assign(s1, s1 + l1);
+  c->last3[1].assign(s2, s2 + l2);
+  c->last3[2].assign(s3, s3 + l3);
+  return copy_triple_outputs(c, l1, l2, l3, outlen, blk, r1, r2, r3, cap, len);
+}
+
+int poyhip_median_3(poyhip_ctx* c, const uint8_t* a, const uint8_t* b, const uint8_t* s3,
+                    int len, uint8_t* out, int cap, int* out_len) {
+  if (!c || !a || !b || !s3 || !out || !out_len || len < 0) return POYHIP_EINVAL;
+  if (!c->has_cm3) return set_err(c, POYHIP_ESTATE, "no 3D cost matrix installed");
+  if (cap < len) return set_err(c, POYHIP_ECAP, "capacity %d < %d", cap, len);
+  (void)hipSetDevice(c->device);
+  *out_len = len;
+  if (len == 0) return POYHIP_OK;
+  const size_t need = 3 * (size_t)len;
+  int rc = grow(c, c->stage, need);
+  if (rc) return rc;
+  if ((rc = grow(c, c->outblk, (size_t)len))) return rc;
+  c->h_stage.resize(need);
+  memcpy(c->h_stage.data(), a, len);
+  memcpy(c->h_stage.data() + len, b, len);
+  memcpy(c->h_stage.data() + 2 * (size_t)len, s3, len);
+  HIPCHK(c, hipMemcpyAsync(c->stage.p, c->h_stage.data(), need, hipMemcpyHostToDevice, c->stream));
+  const uint8_t* d = static_cast<const uint8_t*>(c->stage.p);
+  hipLaunchKernelGGL(k_median_3, dim3(1), dim3(256), 0, c->stream, c->d_cm3, d, d + len,
+                     d + 2 * (size_t)len, len, static_cast<uint8_t*>(c->outblk.p));
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(out, c->outblk.p, len, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return POYHIP_OK;
+}
+
+}  // extern "C"
+
+// Batches of triples
+
+struct poyhip_batch3 {
+  poyhip_ctx* ctx = nullptr;
+  int op = 0;
+  int n = 0;
+  std::vector<int> l1, l2, l3, order;
+  std::vector<int64_t> dm_off, out_off;
+  struct Chunk {
+    int begin, count;
+    int64_t dm_bytes;
+  };
+  std::vector<Chunk> chunks;
+  void* d_tab = nullptr;
+  int64_t *d_dm_off = nullptr, *d_out_off = nullptr;
+  int *d_l1 = nullptr, *d_l2 = nullptr, *d_l3 = nullptr, *d_order = nullptr;
+  int* d_counters = nullptr;
+  uint8_t* d_dm = nullptr;
+  int* d_ring = nullptr;
+  int64_t dm_cap = 0;
+  long long ring_stride = 0;
+  int grid = 0;
+  size_t out_bytes = 0;
+  std::vector<hipEvent_t> ev;
+  float fill_ms = -1.f, total_ms = -1.f;
+};
+
+namespace {
+
+void batch3_free(poyhip_batch3* b) {
+  for (void* p : {(void*)b->d_tab, (void*)b->d_counters, (void*)b->d_dm, (void*)b->d_ring})
+    if (p) (void)hipFree(p);
+  b->d_tab = nullptr;
+  b->d_counters = nullptr;
+  b->d_dm = nullptr;
+  b->d_ring = nullptr;
+  for (hipEvent_t e : b->ev) (void)hipEventDestroy(e);
+  b->ev.clear();
+}
+
+}  // namespace
+
+extern "C" {
+
+int poyhip_batch3_create(poyhip_ctx* c, int op, int n, const int* len1, const int* len2,
+                         const int* len3, size_t dm_budget, poyhip_batch3** out) {
+  if (!c || !out || n <= 0 || !len1 || !len2 || !len3) return POYHIP_EINVAL;
+  if (op != POYHIP_OP_ALIGN_3D && op != POYHIP_OP_COST_3D)
+    return set_err(c, POYHIP_EINVAL, "unknown triple batch op %d", op);
+  if (!c->has_cm3) return set_err(c, POYHIP_ESTATE, "no 3D cost matrix installed");
+  (void)hipSetDevice(c->device);
+  *out = nullptr;
+  poyhip_batch3* b = new poyhip_batch3();
+  b->ctx = c;
+  b->op = op;
+  b->n = n;
+  b->l1.assign(len1, len1 + n);
+  b->l2.assign(len2, len2 + n);
+  b->l3.assign(len3, len3 + n);
+  b->dm_off.resize(n);
+  b->out_off.resize(n);
+  std::vector<double> work(n);
+  for (int k = 0; k < n; k++) {
+    if (len1[k] < 1 || len2[k] < 1 || len3[k] < 1) {
+      delete b;
+      return set_err(c, POYHIP_EINVAL, "triple %d has an empty sequence", k);
+    }
+    work[k] = (double)len1[k] * len2[k] * len3[k];
+    b->ring_stride = std::max<long long>(b->ring_stride, cube_ring_ints(len1[k], len2[k], len3[k]));
+  }
+  b->ring_stride = (b->ring_stride + 63) & ~63LL;
+  b->order.resize(n);
+  std::iota(b->order.begin(), b->order.end(), 0);
+  std::stable_sort(b->order.begin(), b->order.end(),
+                   [&](int x, int y) { return work[x] > work[y]; });
+  int64_t o = 0;
+  for (int k = 0; k < n; k++) {
+    b->out_off[k] = o;
+    if (op == POYHIP_OP_ALIGN_3D) o += (cube_out_bytes(len1[k], len2[k], len3[k]) + 15) & ~(int64_t)15;
+  }
+  b->out_bytes = (size_t)o;
+  size_t fr = 0, tot = 0;
+  if (hipMemGetInfo(&fr, &tot) != hipSuccess) fr = (size_t)8 << 30;
+  // resident wavefronts: up to 8 per CU, with the rings within a quarter of
+  // the free memory
+  long long grid = std::min<long long>(n, (long long)c->num_cus * 8);
+  const long long ring_cap = (long long)(fr / 4) / (b->ring_stride * 4);
+  if (grid > ring_cap) grid = ring_cap;
+  if (grid < 1) {
+    delete b;
+    return set_err(c, POYHIP_ENOMEM, "ring of %lld ints does not fit", b->ring_stride);
+  }
+  b->grid = (int)grid;
+  if (op == POYHIP_OP_ALIGN_3D) {
+    if (dm_budget == 0) dm_budget = std::min(fr / 2, (size_t)160 << 30);
+    int64_t cur = 0;
+    int begin = 0;
+    for (int q = 0; q < n; q++) {
+      const int k = b->order[q];
+      const int64_t sz = (cube_dm_bytes(len1[k], len2[k], len3[k]) + 255) & ~(int64_t)255;
+      if (sz > (int64_t)dm_budget) {
+        delete b;
+        return set_err(c, POYHIP_ECAP, "triple %d needs %lld direction bytes", k, (long long)sz);
+      }
+      if (cur > 0 && cur + sz > (int64_t)dm_budget) {
+        b->chunks.push_back({begin, q - begin, cur});
+        begin = q;
+        cur = 0;
+      }
+      b->dm_off[k] = cur;
+      cur += sz;
+    }
+    b->chunks.push_back({begin, n - begin, cur});
+    for (const auto& ch : b->chunks) b->dm_cap = std::max(b->dm_cap, ch.dm_bytes);
+  } else {
+    b->chunks.push_back({0, n, 0});
+  }
+  auto fail = [&](int code, const char* what) {
+    batch3_free(b);
+    delete b;
+    return set_err(c, code, "triple batch allocation failed: %s", what);
+  };
+  const size_t tab_bytes = (size_t)n * (2 * 8 + 4 * 4) + 1024;
+  if (hipMalloc(&b->d_tab, tab_bytes) != hipSuccess) return fail(POYHIP_ENOMEM, "tables");
+  char* p = static_cast<char*>(b->d_tab);
+  b->d_dm_off = reinterpret_cast<int64_t*>(p); p += 8 * (size_t)n;
+  b->d_out_off = reinterpret_cast<int64_t*>(p); p += 8 * (size_t)n;
+  b->d_l1 = reinterpret_cast<int*>(p); p += 4 * (size_t)n;
+  b->d_l2 = reinterpret_cast<int*>(p); p += 4 * (size_t)n;
+  b->d_l3 = reinterpret_cast<int*>(p); p += 4 * (size_t)n;
+  b->d_order = reinterpret_cast<int*>(p); p += 4 * (size_t)n;
+  if (hipMalloc(&b->d_counters, sizeof(int) * (b->chunks.size() + 2)) != hipSuccess)
+    return fail(POYHIP_ENOMEM, "counters");
+  if (op == POYHIP_OP_ALIGN_3D && hipMalloc(&b->d_dm, (size_t)b->dm_cap + 256) != hipSuccess)
+    return fail(POYHIP_ENOMEM, "direction store");
+  if (hipMalloc(&b->d_ring, sizeof(int) * (size_t)b->ring_stride * b->grid) != hipSuccess)
+    return fail(POYHIP_ENOMEM, "rings");
+  hipStream_t st = c->stream;
+  bool ok = true;
+  ok &= hipMemcpyAsync(b->d_dm_off, b->dm_off.data(), 8 * (size_t)n, hipMemcpyHostToDevice, st) == hipSuccess;
+  ok &= hipMemcpyAsync(b->d_out_off, b->out_off.data(), 8 * (size_t)n, hipMemcpyHostToDevice, st) == hipSuccess;
+  ok &= hipMemcpyAsync(b->d_l1, b->l1.data(), 4 * (size_t)n, hipMemcpyHostToDevice, st) == hipSuccess;
+  ok &= hipMemcpyAsync(b->d_l2, b->l2.data(), 4 * (size_t)n, hipMemcpyHostToDevice, st) == hipSuccess;
+  ok &= hipMemcpyAsync(b->d_l3, b->l3.data(), 4 * (size_t)n, hipMemcpyHostToDevice, st) == hipSuccess;
+  ok &= hipMemcpyAsync(b->d_order, b->order.data(), 4 * (size_t)n, hipMemcpyHostToDevice, st) == hipSuccess;
+  ok &= hipStreamSynchronize(st) == hipSuccess;
+  if (!ok) return fail(POYHIP_EDEVICE, "upload");
+  b->ev.resize(2 * b->chunks.size() + 2);
+  for (auto& e : b->ev)
+    if (hipEventCreate(&e) != hipSuccess) return fail(POYHIP_EDEVICE, "events");
+  *out = b;
+  return POYHIP_OK;
+}
+
+int poyhip_batch3_destroy(poyhip_batch3* b) {
+  if (!b) return POYHIP_EINVAL;
+  (void)hipSetDevice(b->ctx->device);
+  (void)hipStreamSynchronize(b->ctx->stream);
+  batch3_free(b);
+  delete b;
+  return POYHIP_OK;
+}
+
+size_t poyhip_batch3_out_bytes(const poyhip_batch3* b) { return b ? b->out_bytes : 0; }
+
+int64_t poyhip_batch3_out_offset(const poyhip_batch3* b, int k) {
+  if (!b || k < 0 || k >= b->n) return -1;
+  return b->out_off[k];
+}
+
+int poyhip_batch3_out_cap(const poyhip_batch3* b, int k) {
+  if (!b || k < 0 || k >= b->n || b->op != POYHIP_OP_ALIGN_3D) return -1;
+  return (int)cube_out_cap(b->l1[k], b->l2[k], b->l3[k]);
+}
+
+int poyhip_batch3_chunks(const poyhip_batch3* b) { return b ? (int)b->chunks.size() : POYHIP_EINVAL; }
+
+float poyhip_batch3_last_fill_ms(const poyhip_batch3* b) { return b ? b->fill_ms : -1.f; }
+
+int poyhip_batch3_run(poyhip_batch3* b, const uint8_t* d_blob, const int64_t* d_off1,
+                      const int64_t* d_off2, const int64_t* d_off3, int* d_cost, uint8_t* d_out,
+                      int* d_outlen, int sync) {
+  if (!b || !d_blob || !d_off1 || !d_off2 || !d_off3 || !d_cost) return POYHIP_EINVAL;
+  poyhip_ctx* c = b->ctx;
+  if (b->op == POYHIP_OP_ALIGN_3D && (!d_out || !d_outlen))
+    return set_err(c, POYHIP_EINVAL, "align batches need output buffers");
+  (void)hipSetDevice(c->device);
+  hipStream_t st = c->stream;
+  const int nch = (int)b->chunks.size();
+  HIPCHK(c, hipEventRecord(b->ev[2 * nch], st));
+  hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, st, b->d_counters, nch);
+  for (int k = 0; k < nch; k++) {
+    const auto& ch = b->chunks[k];
+    TripleTable t;
+    t.blob = d_blob;
+    t.off1 = d_off1;
+    t.off2 = d_off2;
+    t.off3 = d_off3;
+    t.len1 = b->d_l1;
+    t.len2 = b->d_l2;
+    t.len3 = b->d_l3;
+    t.dm_off = b->d_dm_off;
+    t.out_off = b->d_out_off;
+    t.order = b->d_order + ch.begin;
+    t.n = ch.count;
+    t.ntriples = b->n;
+    CubeFillArgs A;
+    A.tt = t;
+    A.cm3 = c->d_cm3;
+    A.dm = b->d_dm;
+    A.ring = b->d_ring;
+    A.ring_stride = b->ring_stride;
+    A.cost = d_cost;
+    A.counter = b->d_counters + k;
+    const int g = std::min(b->grid, ch.count);
+    HIPCHK(c, hipEventRecord(b->ev[2 * k], st));
+    if (b->op == POYHIP_OP_ALIGN_3D)
+      hipLaunchKernelGGL(k_cube_fill<true>, dim3(g), dim3(64), 0, st, A);
+    else
+      hipLaunchKernelGGL(k_cube_fill<false>, dim3(g), dim3(64), 0, st, A);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventRecord(b->ev[2 * k + 1], st));
+    if (b->op == POYHIP_OP_ALIGN_3D) {
+      CubeBtArgs B;
+      B.tt = t;
+      B.cm3 = c->d_cm3;
+      B.dm = b->d_dm;
+      B.out = d_out;
+      B.outlen = d_outlen;
+      hipLaunchKernelGGL(k_cube_backtrack, dim3((ch.count + 63) / 64), dim3(64), 0, st, B);
+      HIPCHK(c, hipGetLastError());
+    }
+  }
+  HIPCHK(c, hipEventRecord(b->ev[2 * nch + 1], st));
+  if (sync) {
+    HIPCHK(c, hipStreamSynchronize(st));
+    float tot = 0.f;
+    for (int k = 0; k < nch; k++) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, b->ev[2 * k], b->ev[2 * k + 1]) == hipSuccess) tot += ms;
+    }
+    b->fill_ms = tot;
+    float all = 0.f;
+    if (hipEventElapsedTime(&all, b->ev[2 * nch], b->ev[2 * nch + 1]) == hipSuccess)
+      b->total_ms = all;
+  }
+  return POYHIP_OK;
+}
 
 }  // extern "C"

@@ -40,4 +40,22 @@ POYHIP_HD inline int64_t lin_dm_bytes(int La, int Lb) {
 POYHIP_HD inline int64_t lin_out_cap(int La, int Lb) { return (int64_t)La + Lb + 1; }
 POYHIP_HD inline int64_t lin_out_bytes(int La, int Lb) { return 4 * lin_out_cap(La, Lb); }
 
+// ---- 3D cube --------------------------------------------------------------
+// rows are cut into segments of 1024 cells (16 per lane)
+POYHIP_HD inline int cube_nseg(int L3) { return (L3 + 1023) / 1024; }
+POYHIP_HD inline int64_t cube_row_ints(int L3) { return (int64_t)cube_nseg(L3) * 1024; }
+// ring of the last L1+L2 rows, per resident wavefront
+POYHIP_HD inline int64_t cube_ring_ints(int L1, int L2, int L3) {
+  return (int64_t)(L1 + L2) * cube_row_ints(L3);
+}
+// one nibble per cell, 512 bytes per row segment
+POYHIP_HD inline int64_t cube_dm_bytes(int L1, int L2, int L3) {
+  return (int64_t)L1 * L2 * cube_nseg(L3) * 512;
+}
+// outputs: r1, r2, r3, median_3, each L1+L2+L3 bytes
+POYHIP_HD inline int64_t cube_out_cap(int L1, int L2, int L3) { return (int64_t)L1 + L2 + L3; }
+POYHIP_HD inline int64_t cube_out_bytes(int L1, int L2, int L3) {
+  return 4 * cube_out_cap(L1, L2, L3);
+}
+
 }  // namespace poyhip

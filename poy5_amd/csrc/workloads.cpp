@@ -105,7 +105,63 @@ void gen_pair(int config, uint64_t seed, int k, std::vector<uint8_t>& a, std::ve
   mutate(r, root, pr, b);
 }
 
+// Generates triple k of configuration 5: root U[200,1000], three branches
+// each with its own divergence U[3%,15%] at the 10:3 substitution:indel
+// ratio, no IUPAC codes.
+void gen_triple(uint64_t seed, int k, std::vector<uint8_t> out[3]) {
+  SplitMix64 r(seed ^ (0xA24BAED4963EE407ULL * (uint64_t)(k + 1)));
+  r.next();
+  const int L = 200 + r.below(801);
+  std::vector<uint8_t> root(L);
+  for (int t = 0; t < L; t++) root[t] = kBases[r.below(4)];
+  for (int u = 0; u < 3; u++) {
+    const double d = 0.03 + 0.12 * r.uniform();
+    const Params pr{d * 10.0 / 13.0, d * 3.0 / 13.0, 0.0};
+    mutate(r, root, pr, out[u]);
+  }
+}
+
 }  // namespace
+
+extern "C" int poyhip_gen_triples(int config, int n, uint64_t seed, uint8_t* blob, int64_t* off,
+                                  int* len, int64_t* total_bytes) {
+  if (n <= 0 || !total_bytes || config != 5) return POYHIP_EINVAL;
+  const int nt = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
+  std::vector<int> ln(3 * (size_t)n);
+  auto lengths = [&](int t) {
+    std::vector<uint8_t> s[3];
+    for (int k = t; k < n; k += nt) {
+      gen_triple(seed, k, s);
+      for (int u = 0; u < 3; u++) ln[3 * (size_t)k + u] = (int)s[u].size();
+    }
+  };
+  {
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; t++) th.emplace_back(lengths, t);
+    for (auto& x : th) x.join();
+  }
+  int64_t tot = 0;
+  std::vector<int64_t> o(3 * (size_t)n);
+  for (size_t q = 0; q < 3 * (size_t)n; q++) {
+    o[q] = tot;
+    tot += ln[q];
+  }
+  *total_bytes = tot;
+  if (off) memcpy(off, o.data(), sizeof(int64_t) * 3 * (size_t)n);
+  if (len) memcpy(len, ln.data(), sizeof(int) * 3 * (size_t)n);
+  if (!blob) return POYHIP_OK;
+  auto fill = [&](int t) {
+    std::vector<uint8_t> s[3];
+    for (int k = t; k < n; k += nt) {
+      gen_triple(seed, k, s);
+      for (int u = 0; u < 3; u++) memcpy(blob + o[3 * (size_t)k + u], s[u].data(), s[u].size());
+    }
+  };
+  std::vector<std::thread> th;
+  for (int t = 0; t < nt; t++) th.emplace_back(fill, t);
+  for (auto& x : th) x.join();
+  return POYHIP_OK;
+}
 
 extern "C" int poyhip_gen_pairs(int config, int n, uint64_t seed, uint8_t* blob, int64_t* off,
                                 int* len, int64_t* total_bytes) {

@@ -13,7 +13,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from .cost_matrix import GAP, TwoD
+from .cost_matrix import GAP, ThreeD, TwoD
 
 
 def _u8(a):
@@ -30,6 +30,7 @@ class Context:
         _lib.check(self._lib.poyhip_create(device, C.byref(h)))
         self.h = h
         self.cm: TwoD | None = None
+        self.cm3: ThreeD | None = None
 
     def close(self):
         if self.h:
@@ -46,6 +47,11 @@ class Context:
         self._cm_c = cm.to_c()
         _lib.check(self._lib.poyhip_set_cm(self.h, C.byref(self._cm_c)), self.h)
         self.cm = cm
+
+    def set_cm3(self, cm3: ThreeD):
+        self._cm3_c = cm3.to_c()
+        _lib.check(self._lib.poyhip_set_cm3(self.h, C.byref(self._cm3_c)), self.h)
+        self.cm3 = cm3
 
     def chk(self, rc):
         return _lib.check(rc, self.h)
@@ -216,3 +222,54 @@ class Align:
             return self.align_affine_3(a, b)[0]
         ap, bp, _ = self.align_2(a, b)
         return self.median_2(ap, bp)
+
+    # -- three sequences (sequence.ml:940-968) ------------------------------
+    def cost_3(self, s1, s2, s3, uk: int = 0) -> int:
+        """Align.cost_3 / algn_CAML_simple_3."""
+        a, ap = _u8(s1)
+        b, bp = _u8(s2)
+        c, cp = _u8(s3)
+        out = C.c_int()
+        self.ctx.chk(self.L.poyhip_cost_3(self.ctx.h, ap, len(a), bp, len(b), cp, len(c), uk,
+                                          C.byref(out)))
+        return out.value
+
+    def _outs3(self, cap):
+        bufs = [np.zeros(cap, np.uint8) for _ in range(3)]
+        return bufs, [x.ctypes.data_as(C.POINTER(C.c_uint8)) for x in bufs]
+
+    def extract_edited_3(self, s1, s2, s3):
+        """Align.extract_edited_3 / algn_CAML_backtrack_3d after cost_3."""
+        a, ap = _u8(s1)
+        b, bp = _u8(s2)
+        c, cp = _u8(s3)
+        cap = len(a) + len(b) + len(c)
+        bufs, P = self._outs3(cap)
+        n = C.c_int()
+        self.ctx.chk(self.L.poyhip_backtrack_3d(self.ctx.h, ap, len(a), bp, len(b), cp, len(c),
+                                                P[0], P[1], P[2], cap, C.byref(n)))
+        return tuple(x[:n.value].copy() for x in bufs)
+
+    def align_3(self, s1, s2, s3, uk: int = 0):
+        """Align.c_align_3 / algn_CAML_align_3d: ((s1', s2', s3'), cost)."""
+        a, ap = _u8(s1)
+        b, bp = _u8(s2)
+        c, cp = _u8(s3)
+        cap = len(a) + len(b) + len(c)
+        bufs, P = self._outs3(cap)
+        n, cost = C.c_int(), C.c_int()
+        self.ctx.chk(self.L.poyhip_align_3d(self.ctx.h, ap, len(a), bp, len(b), cp, len(c), uk,
+                                            P[0], P[1], P[2], cap, C.byref(n), C.byref(cost)))
+        return tuple(x[:n.value].copy() for x in bufs), cost.value
+
+    def median_3(self, a, b, c):
+        """Align.c_median_3 / algn_CAML_median_3 on aligned sequences."""
+        x, xp = _u8(a)
+        y, yp = _u8(b)
+        z, zp = _u8(c)
+        out = np.zeros(max(len(x), 1), np.uint8)
+        n = C.c_int()
+        self.ctx.chk(self.L.poyhip_median_3(self.ctx.h, xp, yp, zp, len(x),
+                                            out.ctypes.data_as(C.POINTER(C.c_uint8)), len(out),
+                                            C.byref(n)))
+        return out[:n.value].copy()

@@ -124,3 +124,15 @@ def test_3d_tables():
     for a in SINGLES:
         for b in SINGLES:
             assert e(a, a, b)[0] == c2c[(a << 5) + b]
+
+
+@pytest.mark.parametrize("tcm", [(1, 1), (1, 2), (2, 1)])
+def test_product_3d_tables_match_oracle(tcm):
+    o = oracle.cm3_from_2d(oracle.cm_dna(*tcm))
+    p = cost_matrix.three_d_of_two_dim_comb(cost_matrix.of_transformations_and_gaps(*tcm))
+    # the oracle restates the metric branch of of_two_dim_comb only
+    assert cost_matrix.is_metric(cost_matrix.of_transformations_and_gaps(*tcm))
+    idx = np.array([(((i << 5) + j) << 5) + k for i in range(1, 32) for j in range(1, 32)
+                    for k in range(1, 32)])
+    assert np.array_equal(np.array(o.cost)[idx], p.cost[idx])
+    assert np.array_equal(np.array(o.median)[idx], p.median[idx])
