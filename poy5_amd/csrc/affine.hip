@@ -173,12 +173,279 @@ __global__ void k_affine_classify(PairTable pt, uint8_t* cls) {
   cls[pid] = (any & kGap) ? 0 : 1;
 }
 
+// ---------------------------------------------------------------------------
+// Rotating stripe schedule (gap-free pairs)
+//
+// With one wavefront per 64-row stripe, lane t works on row i0+t from step
+// 2t to 2t+W-1 of a stripe that lasts W+126 steps (W = band width), so a
+// wavefront idles on almost half its lane-steps for the band widths of
+// typical pairs.  Here every lane moves on to its row of the next stripe as
+// soon as the schedule allows: stripe b starts at global step S_b and lane t
+// evaluates column j of its row at step S_b + (j - jlo_b) + t.  S_{b+1} is
+// the smallest start that keeps (1) lane 0 of stripe b+1 behind lane 63 of
+// stripe b on every column and (2) every lane's row of stripe b+1 after its
+// row of stripe b.  Lanes 1..63 still take the row above from lane t-1
+// through DPP; lane 0 takes it from a FIFO in LDS that lane 63 fills one
+// stripe earlier (indexed by column mod kRotFifo).  Every lane loads its own
+// column symbols: lanes of different stripes work on unrelated columns, so
+// the symbol cannot travel down the wavefront as in the plain loop.  A round whose schedule
+// needs a deeper FIFO, or more stripes than kRotMaxStripes, runs the plain
+// stripe loop instead.
+//
+// Direction words are laid out [global step / 4][lane] for the whole round
+// and stab[b] = S_b, so the backtrace finds cell (row i0_b + t, column j) of
+// stripe b at global step S_b + (j - jlo_b) + t, exactly as for the plain
+// loop (whose S_b are the running sums of its stripe lengths).
+constexpr int kRotFifo = 512;        // entries (power of two)
+constexpr int kRotMaxStripes = 128;
+
+struct RotShared {
+  int4 fifo[kRotFifo];               // V, CB, H, gap counts of row i0_b - 1
+  int S[kRotMaxStripes + 1];         // stripe start steps
+  int lag;                           // largest FIFO lag of the round
+};
+
+// Band of row i in the round: [lo, hi], right edge rb (src/algn.c:2113-2172).
+__device__ __forceinline__ int band_lo(int i, int newk) { return imax(i - newk, 0); }
+__device__ __forceinline__ int band_hi(int i, int D, int newk, int lenj) {
+  const int rb = i + D + newk;
+  return rb <= lenj - 1 ? rb : lenj;
+}
+
+// Plans the round: S_b for every stripe.  Returns the total number of
+// steps, or -1 when the round must use the plain stripe loop.
+//   C1: lane 0 of stripe b+1 reads column j from the FIFO at least 4 steps
+//       after lane 63 of stripe b wrote it (a 4-step block reads its FIFO
+//       entries up front): S_{b+1} >= S_b + 67 + (jlo_{b+1} - jlo_b).
+//   C2: lane t enters stripe b+1 after its own last cell of stripe b.
+//       Lane t+1 reads lane t's outputs through DPP one step later, so it
+//       still sees lane t's last cell of stripe b; at its own last cell
+//       (the band's right edge) it ignores the row above.
+// The FIFO is safe while lag = S_{b+1} - S_b - (jlo_{b+1} - jlo_b) - 63
+// stays within kRotFifo - 72 (no entry is overwritten before it is read).
+__device__ int rot_plan(RotShared& sh, int leni, int lenj, int D, int newk, int nst,
+                        long long dm_steps_cap) {
+  const int lane = threadIdx.x;
+  if (nst > kRotMaxStripes) return -1;
+  for (int b = lane; b < nst; b += 64) {
+    const int i0 = 1 + (b << 6);
+    const int jlo = band_lo(i0, newk);
+    int d, lag = 0;
+    if (b + 1 < nst) {
+      const int jlon = band_lo(i0 + 64, newk);
+      int c2 = 0;
+      for (int t = 0; t < 64 && i0 + t + 64 <= leni; t++)
+        c2 = imax(c2, band_hi(i0 + t, D, newk, lenj) - band_lo(i0 + t + 64, newk));
+      c2 += (jlon - jlo) + 1;
+      d = imax(67 + (jlon - jlo), c2);
+      lag = d - (jlon - jlo) - 63;
+    } else {
+      d = 0;
+      for (int t = 0; t < 64 && i0 + t <= leni; t++)
+        d = imax(d, band_hi(i0 + t, D, newk, lenj) - jlo + t + 1);
+    }
+    sh.S[b + 1] = d;
+    sh.fifo[b].x = lag;  // scratch: the FIFO is idle between rounds
+  }
+  __syncthreads();
+  if (lane == 0) {
+    int acc = 0, mlag = 0;
+    sh.S[0] = 0;
+    for (int b = 0; b < nst; b++) {
+      acc += sh.S[b + 1];
+      sh.S[b + 1] = acc;
+      mlag = imax(mlag, sh.fifo[b].x);
+    }
+    sh.lag = mlag;
+  }
+  __syncthreads();
+  const int total = __builtin_amdgcn_readfirstlane(sh.S[nst]);
+  const int lag = __builtin_amdgcn_readfirstlane(sh.lag);
+  __syncthreads();
+  if (lag > kRotFifo - 72 || (long long)((total + 3) >> 2) * 4 > dm_steps_cap) return -1;
+  return total;
+}
+
+// One round of the gap-free fill on the rotating schedule.  Same outputs as
+// the plain stripe loop of k_affine_fill: cost and gap count of the last
+// cell, cells and lane-steps, stripe table and direction words.
+__device__ void rot_round_fast(RotShared& sh, int total, const uint8_t* __restrict__ sa,
+                               const uint8_t* __restrict__ sb, int leni, int lenj, int D, int newk,
+                               int nst, int go, int sw, int HA0, const int* s_cost,
+                               const int* s_prep, uint8_t* dm, long long* stab, int& rc_out,
+                               int& rgn_out, long long& cells, long long& slots) {
+  const int lane = threadIdx.x;
+  const int kNever = 0x7fffffff;
+  for (int b = lane; b <= nst; b += 64) stab[b] = sh.S[b];
+  // ---- per-lane state: stripe bt, row i ------------------------------------
+  int bt = 0;
+  int i = 1 + lane;
+  const bool act0 = i <= leni;
+  const int jlo_t = band_lo(1, newk);
+  int my_lo = act0 ? band_lo(i, newk) : (1 << 30);
+  int my_rb = i + D + newk;
+  int my_hi = act0 ? band_hi(i, D, newk, lenj) : -1;
+  if (act0) cells += my_hi - my_lo + 1;
+  // row symbols of stripes b (register b & 1), loaded one stripe ahead when
+  // lane 0 enters stripe b-1
+  int symA = sa[imin(1 + lane, leni)];
+  int symB = nst > 1 ? sa[imin(65 + lane, leni)] : 0;
+  AffRow row;
+  row.ge = s_cost[(symA << 5) + kGap];
+  row.cost_row = (symA & 15) << 5;
+  // column of the lane at step st: st + cbc (current stripe), st + cbn
+  // (next stripe, from step nstart on)
+  int cbc = jlo_t - lane, cbn = 0;
+  int nstart = kNever;
+  if (nst > 1 && i + 64 <= leni) {
+    nstart = sh.S[1] + (band_lo(i + 64, newk) - band_lo(65, newk) + lane);
+    cbn = band_lo(65, newk) - sh.S[1] - lane;
+  }
+  // ---- lane 0's stripe (uniform) --------------------------------------------
+  int b0 = 0, S0 = 0, jlo0 = jlo_t;
+  int next0 = nst > 1 ? __builtin_amdgcn_readfirstlane(sh.S[1]) : kNever;
+  int jlon0 = band_lo(65, newk);
+  const int blast = nst - 1;
+  const int s_fin = __builtin_amdgcn_readfirstlane(sh.S[blast]) +
+                    (lenj - band_lo(1 + (blast << 6), newk)) + ((leni - 1) & 63);
+  int rc = 0, rgn = 0;
+  // up-left cell of lane 0's first step: row 0 at column jlo0 - 1
+  int dV = kHigh, dCB = kHigh, dH = kHigh, dG = 0;
+  if (jlo0 >= 1) {
+    dCB = 0;
+    dH = HA0;
+  }
+  int lH = kHigh, lCB = kHigh, lG = 0;
+  int oV = kHigh, oCB = kHigh, oH = kHigh, oG = 0;
+  int cb0run = 0;
+  int c0prev = sb[imax(jlo0 - 1, 0)];
+  slots += (long long)total * 64;
+  const int total4 = (total + 3) & ~3;
+  // lane 0's column at step st (at most one stripe change in 8 steps)
+  auto col0 = [&](int st) { return st >= next0 ? jlon0 + (st - next0) : jlo0 + (st - S0); };
+  // column symbols, loaded one 4-step block ahead: cn for lane 0's column
+  // (row 0 generator), cl for the lane's own column (each lane loads its
+  // own: lanes of different stripes work on unrelated columns)
+  auto coll = [&](int st) { return imin(imax(st + (st >= nstart ? cbn : cbc), 0), lenj); };
+  int cn[4], cl[4];
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    cn[u] = sb[imin(col0(u), lenj)];
+    cl[u] = sb[coll(u)];
+  }
+  for (int s = 0; s < total4; s += 4) {
+    // lane 63's FIFO writes of the previous block precede these reads in
+    // program order; the barrier keeps the compiler from moving them
+    __builtin_amdgcn_wave_barrier();
+    int c0b[4], clb[4];
+    int4 fv[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      c0b[u] = cn[u];
+      clb[u] = cl[u];
+      fv[u] = sh.fifo[col0(s + u) & (kRotFifo - 1)];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      cn[u] = sb[imin(imax(col0(s + 4 + u), 0), lenj)];
+      cl[u] = sb[coll(s + 4 + u)];
+    }
+    uint32_t pack = 0;
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int st = s + u;
+      // lane 0 enters its next stripe (uniform)
+      if (st == next0) {
+        b0++;
+        S0 = next0;
+        jlo0 = jlon0;
+        next0 = b0 + 1 < nst ? __builtin_amdgcn_readfirstlane(sh.S[b0 + 1]) : kNever;
+        jlon0 = band_lo(1 + ((b0 + 1) << 6), newk);
+        // lane 0's up-left cell: row i0 - 1 at column jlo0 - 1; the other
+        // lanes keep theirs
+        int4 v = make_int4(kHigh, kHigh, kHigh, 0);
+        if (jlo0 >= 1) v = sh.fifo[(jlo0 - 1) & (kRotFifo - 1)];
+        if (lane == 0) {
+          dV = v.x; dCB = v.y; dH = v.z; dG = v.w;
+        }
+        if (b0 + 1 < nst) {
+          const int nsym = sa[imin(1 + ((b0 + 1) << 6) + lane, leni)];
+          if ((b0 + 1) & 1) symB = nsym;
+          else symA = nsym;
+        }
+      }
+      // every lane enters its next stripe once (divergent)
+      if (st == nstart) {
+        bt++;
+        i += 64;
+        my_lo = band_lo(i, newk);
+        my_rb = i + D + newk;
+        my_hi = band_hi(i, D, newk, lenj);
+        cells += my_hi - my_lo + 1;
+        const int ic = (bt & 1) ? symB : symA;
+        row.ge = s_cost[(ic << 5) + kGap];
+        row.cost_row = (ic & 15) << 5;
+        cbc = cbn;
+        if (bt + 1 < nst && i + 64 <= leni) {
+          const int Sn = sh.S[bt + 1], jlon = band_lo(i + 64 - lane, newk);
+          nstart = Sn + (band_lo(i + 64, newk) - jlon + lane);
+          cbn = jlon - Sn - lane;
+        } else {
+          nstart = kNever;
+        }
+      }
+      // lane 0's source row: POY's row 0 in stripe 0, else the FIFO
+      const int jl0 = jlo0 + (st - S0);
+      const int c0 = c0b[u];
+      int sV = kHigh, sCB, sH, sG;
+      if (b0 == 0) {
+        const int gr0 = s_prep[c0];
+        const int gop0 = has_gap_opening(jl0, c0prev, c0, go);
+        const int add0 = ((c0prev & kGap) && !(c0 & kGap)) ? gop0 + gr0 : gr0;
+        cb0run = (jl0 == 0) ? 0 : cb0run + add0;
+        sCB = cb0run;
+        sH = (jl0 == 0) ? HA0 : cb0run;
+        sG = jl0 & 0xFFFF;
+      } else {
+        sV = fv[u].x; sCB = fv[u].y; sH = fv[u].z; sG = fv[u].w;
+      }
+      c0prev = c0;
+      const int jc = clb[u];
+      const int uV = wave_shr1(oV, sV);
+      const int uCB = wave_shr1(oCB, sCB);
+      const int uH = wave_shr1(oH, sH);
+      const int uG = wave_shr1(oG, sG);
+      const int j = st + cbc;
+      AffCell o;
+      aff_cell_fast(row, go, sw, j, jc, j == my_lo, j == my_rb, s_cost, s_prep, uV, uCB, uG, dV,
+                    dCB, dH, dG, lH, lCB, lG, o);
+      const bool inb = j >= my_lo && j <= my_hi;
+      if (inb && lane == 63) sh.fifo[j & (kRotFifo - 1)] = make_int4(o.V, o.C, o.H, o.G);
+      if (st == s_fin) {
+        rc = o.f;
+        rgn = o.gmax;
+      }
+      oV = o.V; oCB = o.C; oH = o.H; oG = o.G;
+      lH = o.H; lCB = o.C; lG = o.G;
+      dV = uV; dCB = uCB; dH = uH; dG = uG;
+      pack |= o.code << (8 * u);
+    }
+    reinterpret_cast<uint32_t*>(dm)[((s >> 2) << 6) + lane] = pack;
+  }
+  const int src = (leni - 1) & 63;
+  rc_out = __shfl(rc, src);
+  rgn_out = __shfl(rgn, src);
+}
+
 template <bool kDm, bool kFast>
 __global__ __launch_bounds__(64) void k_affine_fill(AffFillArgs A) {
-  __shared__ int s_cost[1024];
+  // gap-free pairs only index rows 0..15 of the cost table
+  constexpr int kCostN = kFast ? 512 : 1024;
+  __shared__ int s_cost[kCostN];
   __shared__ int s_prep[32];
+  __shared__ RotShared s_rot;
   const int lane = threadIdx.x;
-  for (int k = lane; k < 1024; k += 64) s_cost[k] = A.cm->cost[k];
+  for (int k = lane; k < kCostN; k += 64) s_cost[k] = A.cm->cost[k];
   if (lane < 32) s_prep[lane] = A.cm->prepend[lane];
   __syncthreads();
   const int go = A.cm->gap_open;
@@ -243,7 +510,14 @@ __global__ __launch_bounds__(64) void k_affine_fill(AffFillArgs A) {
       const int nstripes = (leni + 63) >> 6;
       long long dmpos = 0;
       int rc = 0, rgn = 0;
-      for (int b = 0; b < nstripes; b++) {
+      int rot_total = -1;
+      if constexpr (kFast && kDm) {
+        rot_total = rot_plan(s_rot, leni, lenj, D, newk, nstripes, aff_dm_bytes(La, Lb) >> 6);
+        if (rot_total > 0)
+          rot_round_fast(s_rot, rot_total, sa, sb, leni, lenj, D, newk, nstripes, go, sw, HA0,
+                         s_cost, s_prep, dm, stab, rc, rgn, cells, slots);
+      }
+      for (int b = 0; rot_total <= 0 && b < nstripes; b++) {
         const int i0 = 1 + (b << 6);
         const int ilast = imin(i0 + 63, leni);
         const int i = i0 + lane;
@@ -252,7 +526,7 @@ __global__ __launch_bounds__(64) void k_affine_fill(AffFillArgs A) {
         const int jhi = (ilast + D + newk <= lenj - 1) ? ilast + D + newk : lenj;
         const int nsteps = (jhi - jlo + 1) + (ilast - i0);
         const int nsteps4 = (nsteps + 3) & ~3;
-        if (kDm && lane == 0) stab[b] = dmpos;
+        if (kDm && lane == 0) stab[b] = dmpos >> 6;  // stripe start in steps
         slots += (long long)nsteps * 64;
 
         // row constants of lane t (row i); rows past leni get the last row's
@@ -427,7 +701,9 @@ __global__ __launch_bounds__(64) void k_affine_backtrace(AffBtArgs A) {
     }
     const int t = (i - 1) & 63;
     const int s = j - jlo + t;
-    long long at = base + ((long long)(s >> 2) << 8) + (t << 2) + (s & 3);
+    // stab[b] = global step of the stripe's first step; words [step/4][lane]
+    const long long g = base + s;
+    long long at = ((g >> 2) << 8) + (t << 2) + (g & 3);
     // cells on the path are inside the band; the clamp only keeps a corrupt
     // walk inside the pair's region
     if (s < 0 || at >= dm_cap) at = 0;

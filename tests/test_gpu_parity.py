@@ -122,7 +122,11 @@ def test_medians_dropin(ctx):
             assert np.array_equal(al.median_2(o1, o2), oracle.median_2_no_gaps(o1, o2, ocm)), k
             assert np.array_equal(al.median_2_with_gaps(o1, o2),
                                   oracle.median_2_with_gaps(o1, o2, ocm)), k
-            assert al.max_cost_2(o1, o2) == oracle.worst_2(o1, o2, ocm), k
+            # Align.max_cost_2 (sequence.ml:652): 0 when a side is all gaps,
+            # else the C stub algn_CAML_worst_2
+            empty = bool(np.all(o1 == 16) or np.all(o2 == 16))
+            want = 0 if empty else oracle.worst_2(o1, o2, ocm)
+            assert al.max_cost_2(o1, o2) == want, k
 
 
 def test_batch_affine_config2_sample(ctx):
@@ -131,6 +135,27 @@ def test_batch_affine_config2_sample(ctx):
     ocm = oracle.cm_dna(1, 1, 3)
     pairs = workloads.pairs_list(2, 96)
     pairs = [p for p in pairs if len(p[0]) * len(p[1]) < 1_200_000][:48]
+    costs, outs = run_pairs(ctx, "align_affine", pairs)
+    for k, (a, b) in enumerate(pairs):
+        sw = len(a) > len(b)
+        si, sj = (b, a) if sw else (a, b)
+        ref = oracle.align_affine_3(si, sj, ocm, int(sw))
+        ra, rb = (ref["resj"], ref["resi"]) if sw else (ref["resi"], ref["resj"])
+        assert costs[k] == ref["cost"], (k, costs[k], ref["cost"])
+        assert np.array_equal(outs[k][0], ra) and np.array_equal(outs[k][1], rb), k
+        assert np.array_equal(outs[k][2], ref["median"]), k
+        assert np.array_equal(outs[k][3], ref["medianwg"]), k
+
+
+def test_batch_affine_config2_long(ctx):
+    """The longest pairs of a 256-pair draw (many 64-row stripes, several
+    doubling rounds, wide and narrow bands) together with short ones."""
+    say("test_batch_affine_config2_long")
+    ctx.set_cm(cost_matrix.of_transformations_and_gaps(1, 1, 3))
+    ocm = oracle.cm_dna(1, 1, 3)
+    pairs = workloads.pairs_list(2, 256, seed=0x5EED)
+    pairs.sort(key=lambda p: -len(p[0]) * len(p[1]))
+    pairs = pairs[:6] + pairs[-10:]
     costs, outs = run_pairs(ctx, "align_affine", pairs)
     for k, (a, b) in enumerate(pairs):
         sw = len(a) > len(b)
