@@ -42,6 +42,9 @@ def parse():
                     help="CPU baseline runs every N-th pair")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-serve", action="store_true",
+                    help="skip the serving run (128-taxon SPR rejoin traffic)")
+    ap.add_argument("--serve-requests", type=int, default=4000)
     ap.add_argument("--graph", type=int, default=0,
                     help="replay each step as one HIP graph (no per-kernel events then)")
     ap.add_argument("--dm-budget-gb", type=float, default=0.0)
@@ -134,6 +137,35 @@ def plain_per_call(ctx, al, blob, off, ln, sample_n, seed=1):
     return {"gcups": cells / wall / 1e9, "pairs": int(len(idx)), "cells": cells,
             "wall_s": wall, "p50_ms": float(np.percentile(lat, 50)),
             "p99_ms": float(np.percentile(lat, 99))}
+
+
+def serving(n_req):
+    """Continuous batching on the server: down pass of a 128-taxon tree
+    (align_2 + ancestor medians kept on the GPU), then cost_2 rejoin
+    requests from Poisson arrivals at 0.25/0.5/0.9 of the measured
+    capacity (linear tcm sub 1 / indel 2)."""
+    from poy5_amd import cost_matrix, spr
+    from poy5_amd.sequence import Context
+    from poy5_amd.serve import Server
+    ctx = Context(0)
+    ctx.set_cm(cost_matrix.of_transformations_and_gaps(1, 2))
+    srv = Server(ctx, n_slots=3, max_batch_pairs=2048, max_len=4096, slot_dm_bytes=2 << 30,
+                 pool_bytes=64 << 20)
+    try:
+        parent, children, leaves = spr.simulate()
+        t0 = time.perf_counter()
+        mid = spr.downpass(srv, parent, children, leaves)
+        down_s = time.perf_counter() - t0
+        res = spr.run_loads(srv, children, mid, n_req=n_req)
+        res["workload"] = ("128-taxon Yule tree, root 1500, JC + 2% indels; down pass "
+                           "(127 align_2 + ancestor, medians kept in the device pool), then "
+                           "cost_2(clade median, node median) rejoin requests")
+        res["downpass_ms"] = round(down_s * 1e3, 3)
+        res["server"] = srv.stats()
+        return res
+    finally:
+        srv.close()
+        ctx.close()
 
 
 def load_traffic():
@@ -256,6 +288,9 @@ def main():
         result["batched_vs_plain"] = round(gcups / pl["gcups"], 2)
         result["p50_ms"] = round(pl["p50_ms"], 3)
         result["p99_ms"] = round(pl["p99_ms"], 3)
+    if not args.no_serve:
+        print("[bench] serving: 128-taxon SPR rejoin traffic", file=sys.stderr, flush=True)
+        result["serving"] = serving(args.serve_requests)
     if not args.no_cpu:
         print("[bench] CPU baseline", file=sys.stderr, flush=True)
         result["cpu_baseline"] = cpu_baseline(blob, off, ln, args.cpu_stride, gpu_costs)

@@ -232,6 +232,54 @@ int poyhip_batch3_chunks(const poyhip_batch3 *b);
 /* Summed fill duration of the last run with sync set (HIP events). */
 float poyhip_batch3_last_fill_ms(const poyhip_batch3 *b);
 
+/* ---- serving (no reference counterpart) --------------------------------- */
+/* A continuous-batching server over one context: requests arrive one at a
+ * time (a tree search's rejoin and join alignments), a worker thread admits
+ * them into S slots (one HIP stream and preallocated buffers each) while
+ * their direction stores fit a slot's budget, and launches each slot's
+ * requests as one batch; slots overlap on the device.  Sequences live in a
+ * device-resident pool; an align_2d request can keep its ancestor median in
+ * the pool (written by the GPU) for later requests.  Install the cost model
+ * on the context before creating the server and keep it while it runs. */
+#define POYHIP_REQ_KEEP_MEDIAN 1
+typedef struct poyhip_server poyhip_server;
+typedef struct {
+  int n_slots;          /* concurrent batches (default 3)                 */
+  int max_batch_pairs;  /* requests per batch (default 4096)              */
+  int max_len;          /* longest sequence, leading gap included (16384) */
+  size_t slot_dm_bytes; /* direction-store budget per slot (4 GiB)        */
+  size_t pool_bytes;    /* sequence pool (256 MiB)                        */
+} poyhip_server_opts;
+int poyhip_server_create(poyhip_ctx *ctx, const poyhip_server_opts *opts,
+                         poyhip_server **out);
+int poyhip_server_destroy(poyhip_server *s);
+/* Appends a host sequence to the pool (uploaded with the next batch). */
+int poyhip_server_put_seq(poyhip_server *s, const uint8_t *seq, int len, int64_t *id);
+/* Copies a pool sequence (e.g. a kept median) back to the host. */
+int poyhip_server_get_seq(poyhip_server *s, int64_t id, uint8_t *out, int cap, int *len);
+/* Queues one request on pool sequences a and b (caller order), op one of the
+ * POYHIP_OP_* pair ops, deltaw < 0 for Align.cost_2's ?deltaw=None.  With
+ * POYHIP_REQ_KEEP_MEDIAN (align_2d) *median_id names the pool entry that
+ * will hold the ancestor median; requests naming it wait until it exists. */
+int poyhip_server_submit(poyhip_server *s, int op, int64_t a, int64_t b, int deltaw,
+                         int flags, int64_t *req, int64_t *median_id);
+/* Waits for a request (timeout_ms < 0: forever; POYHIP_EBUSY on timeout). */
+int poyhip_server_wait(poyhip_server *s, int64_t req, int timeout_ms);
+/* Result of a finished request: cost, aligned sequences (align ops, caller
+ * order, cap >= their length), latency from submission to completion;
+ * release != 0 forgets the request. */
+int poyhip_server_result(poyhip_server *s, int64_t req, int *cost, uint8_t *r1,
+                         uint8_t *r2, int cap, int *len, double *latency_us, int release);
+int poyhip_server_stats(poyhip_server *s, int64_t *batches, int64_t *pairs,
+                        double *cells_full, int *max_batch, int *max_inflight);
+/* Open-loop load generator: submits request k at arrival_us[k] after the
+ * call from a native thread, then waits for all and reports per-request
+ * costs and latencies (any output array may be NULL) and the wall time. */
+int poyhip_server_replay(poyhip_server *s, int n, const double *arrival_us, const int *ops,
+                         const int64_t *a, const int64_t *b, const int *deltaw,
+                         const int *flags, int *costs, double *latency_us,
+                         int64_t *median_ids, double *wall_us);
+
 /* ---- synchronisation helpers -------------------------------------------- */
 int poyhip_sync(poyhip_ctx *ctx);
 /* Opaque hipStream_t of the context (for callers that enqueue their own

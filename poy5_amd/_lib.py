@@ -41,6 +41,14 @@ class PoyhipCM3(C.Structure):
     _fields_ = [("cost", C.c_int * 32768), ("median", C.c_uint8 * 32768)]
 
 
+class PoyhipServerOpts(C.Structure):
+    _fields_ = [("n_slots", C.c_int), ("max_batch_pairs", C.c_int), ("max_len", C.c_int),
+                ("slot_dm_bytes", C.c_size_t), ("pool_bytes", C.c_size_t)]
+
+
+REQ_KEEP_MEDIAN = 1
+
+
 class PoyhipError(RuntimeError):
     def __init__(self, code, msg=""):
         super().__init__(f"{ERRORS.get(code, code)}: {msg}")
@@ -89,6 +97,19 @@ SIGNATURES = {
     "poyhip_batch3_run": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_int]),
     "poyhip_batch3_chunks": (C.c_int, [_vp]),
     "poyhip_batch3_last_fill_ms": (C.c_float, [_vp]),
+    "poyhip_server_create": (C.c_int, [_vp, C.POINTER(PoyhipServerOpts), C.POINTER(_vp)]),
+    "poyhip_server_destroy": (C.c_int, [_vp]),
+    "poyhip_server_put_seq": (C.c_int, [_vp, _u8p, C.c_int, _i64p]),
+    "poyhip_server_get_seq": (C.c_int, [_vp, C.c_int64, _u8p, C.c_int, _ip]),
+    "poyhip_server_submit": (C.c_int, [_vp, C.c_int, C.c_int64, C.c_int64, C.c_int, C.c_int,
+                                       _i64p, _i64p]),
+    "poyhip_server_wait": (C.c_int, [_vp, C.c_int64, C.c_int]),
+    "poyhip_server_result": (C.c_int, [_vp, C.c_int64, _ip, _u8p, _u8p, C.c_int, _ip,
+                                       C.POINTER(C.c_double), C.c_int]),
+    "poyhip_server_stats": (C.c_int, [_vp, _i64p, _i64p, C.POINTER(C.c_double), _ip, _ip]),
+    "poyhip_server_replay": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_double), _ip, _i64p, _i64p,
+                                       _ip, _ip, _ip, C.POINTER(C.c_double), _i64p,
+                                       C.POINTER(C.c_double)]),
     "poyhip_gen_triples": (C.c_int, [C.c_int, C.c_int, C.c_uint64, _u8p, _i64p, _ip, _i64p]),
     "poyhip_batch_create": (C.c_int, [_vp, C.c_int, C.c_int, _ip, _ip, _ip, C.c_size_t,
                                       C.POINTER(_vp)]),

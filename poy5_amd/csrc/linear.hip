@@ -325,8 +325,10 @@ __global__ void k_linear_deltawh(PairTable pt, int* deltawh, int has_deltaw, con
   for (int k = 0; k < L2; k++) g2 += (sb[k] >= kGap && (sb[k] & kGap)) ? 1 : 0;
   const int dif = L1 - L2;
   const int lower = L1 / 10;
+  // has_deltaw: 0 = ?deltaw=None for every pair, 1 = ~deltaw for every
+  // pair, 2 = per pair (a negative deltaw[pid] means None)
   int dw;
-  if (!has_deltaw) dw = dif < lower ? lower / 2 : 2;
+  if (!has_deltaw || (has_deltaw == 2 && deltaw[pid] < 0)) dw = dif < lower ? lower / 2 : 2;
   else dw = dif < lower ? lower : deltaw[pid];
   deltawh[pid] = imax(g1, g2) + dw;
 }

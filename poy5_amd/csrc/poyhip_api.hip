@@ -1443,3 +1443,5 @@ int poyhip_batch3_run(poyhip_batch3* b, const uint8_t* d_blob, const int64_t* d_
 }
 
 }  // extern "C"
+
+#include "serve.hip"

@@ -116,3 +116,16 @@ def test_bench_refuses_several_gpus():
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2
     assert "single MI355X" in r.stderr
+
+
+def test_spr_workload_deterministic():
+    from poy5_amd import spr
+    p1, c1, l1 = spr.simulate(n_taxa=32, root_len=200, seed=9)
+    p2, c2, l2 = spr.simulate(n_taxa=32, root_len=200, seed=9)
+    assert np.array_equal(p1, p2) and c1 == c2
+    assert len(l1) == 32 and len(p1) == 63
+    for v in l1:
+        assert np.array_equal(l1[v], l2[v]) and l1[v][0] == 16
+        assert np.all((l1[v][1:] >= 1) & (l1[v][1:] <= 8))
+    order = spr.post_order(c1)
+    assert order[-1] == 0 and sorted(order) == list(range(63))
