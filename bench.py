@@ -45,6 +45,10 @@ def parse():
     ap.add_argument("--no-serve", action="store_true",
                     help="skip the serving run (128-taxon SPR rejoin traffic)")
     ap.add_argument("--serve-requests", type=int, default=4000)
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the secondary configurations (1, 4, 5)")
+    ap.add_argument("--extra-long-pairs", type=int, default=512)
+    ap.add_argument("--extra-triples", type=int, default=1024)
     ap.add_argument("--graph", type=int, default=0,
                     help="replay each step as one HIP graph (no per-kernel events then)")
     ap.add_argument("--dm-budget-gb", type=float, default=0.0)
@@ -137,6 +141,71 @@ def plain_per_call(ctx, al, blob, off, ln, sample_n, seed=1):
     return {"gcups": cells / wall / 1e9, "pairs": int(len(idx)), "cells": cells,
             "wall_s": wall, "p50_ms": float(np.percentile(lat, 50)),
             "p99_ms": float(np.percentile(lat, 99))}
+
+
+def _timed(run, steps=2):
+    run()
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        run()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def secondary(args):
+    """GCUPS of the other workloads on full cells, batched, inputs resident:
+    config 1 (1,024 linear pairs, align_2 + ancestor + median with gaps),
+    config 4 (the first N pairs of 16k-100k, linear sub 1 / indel 1, band
+    doubling, align_2 + medians), config 5 (the first N triples, 3D cube +
+    backtrack + median_3)."""
+    import numpy as np
+    import torch
+    from poy5_amd import cost_matrix, workloads
+    from poy5_amd.batch import Batch, Batch3
+    from poy5_amd.sequence import Context
+    dev = torch.device("cuda:0")
+    out = {}
+    for cfg, n, tcm in ((1, 1024, (1, 2)), (4, args.extra_long_pairs, (1, 1))):
+        ctx = Context(0)
+        ctx.set_cm(cost_matrix.of_transformations_and_gaps(*tcm))
+        blob, off, ln = workloads.gen_pairs(cfg, n)
+        l1, l2 = ln[0::2].astype(np.int32), ln[1::2].astype(np.int32)
+        b = Batch(ctx, "align_2d", l1, l2)
+        d_blob = torch.from_numpy(blob).to(dev)
+        d1 = torch.from_numpy(np.ascontiguousarray(off[0::2])).to(dev)
+        d2 = torch.from_numpy(np.ascontiguousarray(off[1::2])).to(dev)
+        cost, o, ol = b.alloc_outputs(dev)
+        dt = _timed(lambda: b.run(d_blob, d1, d2, cost, o, ol, sync=True))
+        cells = float(np.sum(l1.astype(np.float64) * l2))
+        cb, _, rounds, _ = b.stats()
+        out[f"config{cfg}"] = {"pairs": n, "op": "align_2d", "tcm": list(tcm),
+                               "ms": round(dt * 1e3, 3), "cells_full": cells,
+                               "cells_computed": float(cb.sum()),
+                               "gcups": round(cells / dt / 1e9, 2),
+                               "chunks": b.chunks(), "mean_rounds": float(rounds.mean())}
+        b.close()
+        ctx.close()
+    ctx = Context(0)
+    two = cost_matrix.of_transformations_and_gaps(1, 1)
+    ctx.set_cm(two)
+    ctx.set_cm3(cost_matrix.three_d_of_two_dim_comb(two))
+    n3 = args.extra_triples
+    blob, off, ln = workloads.gen_triples(n3)
+    lens = [ln[u::3].astype(np.int32) for u in range(3)]
+    b3 = Batch3(ctx, "align_3d", *lens)
+    d_blob = torch.from_numpy(blob).to(dev)
+    ds = [torch.from_numpy(np.ascontiguousarray(off[u::3])).to(dev) for u in range(3)]
+    cost, o, ol = b3.alloc_outputs(dev)
+    dt = _timed(lambda: b3.run(d_blob, *ds, cost, o, ol, sync=True), steps=1)
+    cells = float(np.sum(lens[0].astype(np.float64) * lens[1] * lens[2]))
+    out["config5"] = {"triples": n3, "op": "align_3d", "ms": round(dt * 1e3, 3),
+                      "cells_full": cells, "gcups": round(cells / dt / 1e9, 2),
+                      "chunks": b3.chunks(), "fill_ms": round(b3.fill_ms(), 3)}
+    b3.close()
+    ctx.close()
+    return out
 
 
 def serving(n_req):
@@ -288,6 +357,9 @@ def main():
         result["batched_vs_plain"] = round(gcups / pl["gcups"], 2)
         result["p50_ms"] = round(pl["p50_ms"], 3)
         result["p99_ms"] = round(pl["p99_ms"], 3)
+    if not args.no_extra:
+        print("[bench] secondary configurations", file=sys.stderr, flush=True)
+        result["secondary"] = secondary(args)
     if not args.no_serve:
         print("[bench] serving: 128-taxon SPR rejoin traffic", file=sys.stderr, flush=True)
         result["serving"] = serving(args.serve_requests)

@@ -124,14 +124,14 @@ class Batch:
         return res
 
 
-def run_pairs(ctx: Context, op: str, pairs, deltaw=None, device="cuda"):
+def run_pairs(ctx: Context, op: str, pairs, deltaw=None, device="cuda", dm_budget: int = 0):
     """Convenience wrapper: host pairs in, host results out.
 
     Returns (costs, outputs) where outputs[k] is the list of four sequences
     of pair k for align ops (caller order: s1', s2', median, median with
     gaps) and None for cost ops."""
     blob, off1, off2, len1, len2 = pack_pairs(pairs)
-    b = Batch(ctx, op, len1, len2, deltaw=deltaw)
+    b = Batch(ctx, op, len1, len2, deltaw=deltaw, dm_budget=dm_budget)
     d_blob = torch.from_numpy(blob).to(device)
     d_off1 = torch.from_numpy(off1).to(device)
     d_off2 = torch.from_numpy(off2).to(device)

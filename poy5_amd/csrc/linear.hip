@@ -18,6 +18,12 @@ struct LinFillArgs {
   long long* stab;
   PairResult* res;
   int* counter;
+  // 0: every doubling round, directions kept from the last one
+  // 1: every doubling round, no directions; the final threshold T is left
+  //    in res[pid].pad for phase 2
+  // 2: one round at the threshold res[pid].pad, directions kept; the
+  //    statistics of phase 1 are kept
+  int phase;
 };
 
 __device__ __forceinline__ int ldc8(const uint8_t* p, int j, int n) {
@@ -66,7 +72,17 @@ __global__ __launch_bounds__(64) void k_linear_fill(LinFillArgs A) {
     int T = (Dd + 1) * delta;
     int rounds = 0, newk = 0, cost = 0, gap_num = 0;
     long long cells = 0, slots = 0;
+    long long cells1 = 0;
+    if (A.phase == 2) {
+      const PairResult r1 = A.res[pid];
+      T = r1.pad;
+      rounds = r1.rounds - 1;
+      cells1 = r1.cells;
+      slots = lane == 0 ? r1.slots : 0;
+    }
+    int Tfin = T;
     for (;;) {
+      Tfin = T;
       const int p = (T - Dd) / 2;
       newk = full ? L1 - 1 : (p >= L1 ? L1 - 1 : p);
       const int lfr = full ? L2 : imin(L2 - L1 + 1 + p, L2);
@@ -81,6 +97,10 @@ __global__ __launch_bounds__(64) void k_linear_fill(LinFillArgs A) {
       }
       const int nstripes = (L1 - 1 + 63) >> 6;
       long long dmpos = 0;
+      // final cell (L1-1, L2-1): last stripe, lane (L1-2) & 63
+      const int blast = nstripes - 1;
+      const int jlo_last = full ? 0 : imax(1 + (blast << 6) - newk, 0);
+      const int s_fin = (L2 - 1) - jlo_last + ((L1 - 2) & 63);
       for (int b = 0; b < nstripes; b++) {
         const int i0 = 1 + (b << 6);
         const int ilast = imin(i0 + 63, L1 - 1);
@@ -89,17 +109,22 @@ __global__ __launch_bounds__(64) void k_linear_fill(LinFillArgs A) {
         const int jlo = full ? 0 : imax(i0 - newk, 0);
         const int jhi = full ? L2 - 1 : imin(ilast + Dd + newk, L2 - 1);
         const int nsteps = (jhi - jlo + 1) + (ilast - i0);
+        const int nsteps16 = (nsteps + 15) & ~15;
         if (kDm && lane == 0) stab[b] = dmpos;
         slots += (long long)nsteps * 64;
-        int s1i = 0, c = 0, tl = 0, col0 = 0, lo = 1 << 30, hi = -1;
+        // row constants; lanes past L1-1 get an empty band
+        const int ir = imin(i, L1 - 1);
+        const int s1i = sa[ir];
+        const int c = s_cost[(s1i << 5) + kGap];
+        const int tl = s_tail[s1i];
+        const int col0 = full ? c : tl;
+        const int arow = s1i << 5;
+        int lo = 1 << 30, hi = -1;
         bool hasL = false, hasR = false;
         if (act) {
-          s1i = sa[i];
-          c = s_cost[(s1i << 5) + kGap];
-          tl = s_tail[s1i];
-          col0 = full ? c : tl;
           if (full) {
-            lo = 0; hi = L2 - 1;
+            lo = 0;
+            hi = L2 - 1;
           } else {
             hasL = (i - newk) > 0;
             lo = hasL ? i - newk : 0;
@@ -108,91 +133,100 @@ __global__ __launch_bounds__(64) void k_linear_fill(LinFillArgs A) {
           }
           cells += hi - lo + 1;
         }
-        const int* rb = bnd + ((b & 1) ^ 1) * 2 * L2;
-        int* wb = bnd + (b & 1) * 2 * L2;
-        int dmv = 0, ddc = 0;  // up-left
-        if (lane == 0 && jlo >= 1) {
-          if (b == 0) { dmv = 0; ddc = 0; }  // row 0, column jlo-1 == 0
-          else { dmv = rb[jlo - 1]; ddc = rb[L2 + jlo - 1]; }
+        const int2* rb = reinterpret_cast<const int2*>(bnd + ((b & 1) ^ 1) * 2 * L2);
+        int2* wb = reinterpret_cast<int2*>(bnd + (b & 1) * 2 * L2);
+        // up-left cell of lane 0's first step: row i0-1 at column jlo-1
+        int dmv = 0, ddc = 0;
+        if (jlo >= 1 && b > 0) {
+          const int2 v = rb[jlo - 1];
+          dmv = v.x;
+          ddc = v.y;
         }
         int lmv = 0, ldc = 0;   // left
         int omv = 0, odc = 0;   // handed to the next lane
-        int r0run = 0;          // row-0 prefix (stripe 0, lane 0)
-        int jc_next = ldc8(sb, jlo - lane, L2);
-        int nmv = 0, ndc = 0;
-        if (lane == 0 && b > 0 && jlo <= L2 - 1) { nmv = rb[jlo]; ndc = rb[L2 + jlo]; }
-        uint32_t pack = 0;
-        for (int s = 0; s < nsteps; s++) {
-          const int j = jlo + s - lane;
-          const int jc = jc_next;
-          jc_next = ldc8(sb, j + 1, L2);
-          int smv = nmv, sdc = ndc;
-          if (lane == 0) {
+        int r0run = 0;          // POY's row 0 at lane 0's column (stripe 0)
+        int jprev = ldc8(sb, jlo - lane - 1, L2);
+        // lane 0's sources of a 16-step block (column symbol, row above),
+        // spread over lanes 0..15 and loaded one block ahead; step u reads
+        // lane u's copy with readlane
+        auto load_src = [&](int s0, int& sym, int& mv, int& dcv) {
+          const int jc0 = imin(jlo + s0 + (lane & 15), L2 - 1);
+          sym = sb[jc0];
+          mv = 0;
+          dcv = 0;
+          if (b > 0) {
+            const int2 v = rb[jc0];
+            mv = v.x;
+            dcv = v.y;
+          }
+        };
+        int nsym, nmv, ndc;
+        load_src(0, nsym, nmv, ndc);
+        for (int s = 0; s < nsteps16; s += 16) {
+          const int csym = nsym, cmv = nmv, cdc = ndc;
+          load_src(s + 16, nsym, nmv, ndc);
+          uint32_t pack = 0;
+#pragma unroll
+          for (int u = 0; u < 16; u++) {
+            const int st = s + u;
+            const int j = jlo + st - lane;
+            const int c0 = __builtin_amdgcn_readlane(csym, u);
+            int smv = __builtin_amdgcn_readlane(cmv, u), sdc = __builtin_amdgcn_readlane(cdc, u);
             if (b == 0) {
-              if (j > 0) r0run += s_prep[jc];
+              // row 0: prefix sums of prepend costs (algn_fill_first_row)
+              if (jlo + st > 0) r0run += s_prep[c0];
               smv = r0run;
               sdc = 0;
-            } else if (j + 1 <= L2 - 1) {
-              nmv = rb[j + 1];
-              ndc = rb[L2 + j + 1];
             }
-          }
-          const int umv = wave_shr1(omv, smv);
-          const int udc = wave_shr1(odc, sdc);
-          uint32_t mv = 0;
-          if (act && j >= lo && j <= hi) {
-            int val, fl;
+            const int jc = wave_shr1(jprev, c0);
+            const int umv = wave_shr1(omv, smv);
+            const int udc = wave_shr1(odc, sdc);
+            // candidates of algn_fill_row / the band edge cells
+            // (src/algn.c:458-533, 543-652); a disabled move is priced out
+            constexpr int kOff = 1 << 29;
+            const int alg = s_cost[arow + jc];
+            const int gapj = s_cost[(kGap << 5) + jc];
+            const bool edge_l = hasL && j == lo;
+            const bool edge_r = hasR && j == hi;
+            const int del = edge_r ? kOff : umv + c;
+            const int ins = edge_l ? kOff : lmv + gapj;
+            const int aln = dmv + alg;
+            int val = imin(imin(del, ins), aln);
+            int fl = (aln == val ? kAlign : 0) | (ins == val ? kInsert : 0) |
+                     (del == val ? kDelete : 0);
+            // algn_fill_last_column (src/algn.c:654)
+            const int cst = (!hasR && j == L2 - 1) ? tl + umv : kOff;
+            fl = cst < val ? kDelete : (fl | (cst == val ? kDelete : 0));
+            val = imin(val, cst);
             if (j == 0) {
               val = umv + col0;
               fl = kDelete;
-            } else {
-              const int alg = s_cost[(s1i << 5) + jc];
-              const int gapj = s_cost[(kGap << 5) + jc];
-              if (hasL && j == lo) {
-                if (hasR && lo == hi) {
-                  val = dmv + alg;
-                  fl = kAlign;
-                } else {
-                  const int del = umv + c, aln = dmv + alg;
-                  val = imin(del, aln);
-                  fl = (del == val ? kDelete : 0) | (aln == val ? kAlign : 0);
-                }
-              } else if (hasR && j == hi) {
-                const int ins = lmv + gapj, aln = dmv + alg;
-                val = imin(ins, aln);
-                fl = (ins == val ? kInsert : 0) | (aln == val ? kAlign : 0);
-              } else {
-                const int del = umv + c, ins = lmv + gapj, aln = dmv + alg;
-                val = imin(imin(del, ins), aln);
-                fl = (aln == val ? kAlign : 0) | (ins == val ? kInsert : 0) | (del == val ? kDelete : 0);
-              }
-              if (!hasR && j == L2 - 1) {
-                // algn_fill_last_column (src/algn.c:654)
-                const int cst = tl + umv;
-                if (cst < val) { val = cst; fl = kDelete; }
-                else if (cst == val) fl |= kDelete;
-              }
             }
             // backtrace_2d_gaps order: align, insert, delete
             const int dc = (fl & kAlign) ? ddc : ((fl & kInsert) ? ldc : udc + 1);
             // backtrace_2d order: align, then delete/insert by swap flag
-            mv = (fl & kAlign) ? 0u
-                 : (sw ? ((fl & kInsert) ? 2u : 1u) : ((fl & kDelete) ? 1u : 2u));
-            omv = val; odc = dc;
-            lmv = val; ldc = dc;
-            if (i == L1 - 1 && j == L2 - 1) { rc = val; rdc = dc; }
-            if (lane == 63) { wb[j] = val; wb[L2 + j] = dc; }
-          }
-          if (kDm) {
-            pack |= mv << ((s & 15) << 1);
-            if ((s & 15) == 15 || s == nsteps - 1) {
-              reinterpret_cast<uint32_t*>(dm + dmpos)[((s >> 4) << 6) + lane] = pack;
-              pack = 0;
+            const uint32_t mvv = (fl & kAlign) ? 0u
+                                 : (sw ? ((fl & kInsert) ? 2u : 1u) : ((fl & kDelete) ? 1u : 2u));
+            const bool inb = act && j >= lo && j <= hi;
+            if (inb) {
+              omv = val;
+              odc = dc;
+              lmv = val;
+              ldc = dc;
+              if (lane == 63) wb[j] = make_int2(val, dc);
             }
+            if (b == blast && st == s_fin) {
+              rc = val;
+              rdc = dc;
+            }
+            pack |= (inb ? mvv : 0u) << (2 * u);
+            dmv = umv;
+            ddc = udc;
+            jprev = jc;
           }
-          dmv = umv; ddc = udc;
+          if (kDm) reinterpret_cast<uint32_t*>(dm + dmpos)[((s >> 4) << 6) + lane] = pack;
         }
-        dmpos += (long long)((nsteps + 15) >> 4) * 256;
+        dmpos += (long long)(nsteps16 >> 4) * 256;
         __threadfence_block();
       }
       if (L1 > 1) {
@@ -205,11 +239,13 @@ __global__ __launch_bounds__(64) void k_linear_fill(LinFillArgs A) {
         gap_num = L2 - 1;
       }
       rounds++;
-      if (full) break;
+      if (full || A.phase == 2) break;
       const int newp = (2 * T - Dd) / 2;
       if ((gap_num + 1) < p || newp - L2 + 1 >= 0) break;
       T *= 2;
     }
+    // phase 2 repeats phase 1's last round: report phase 1's cells
+    if (A.phase == 2) cells = lane == 0 ? cells1 : 0;
     for (int o = 32; o > 0; o >>= 1) cells += __shfl_xor(cells, o);
     if (lane == 0) {
       PairResult r;
@@ -218,7 +254,7 @@ __global__ __launch_bounds__(64) void k_linear_fill(LinFillArgs A) {
       r.newk = newk;
       r.rounds = rounds;
       r.full = full ? 1 : 0;
-      r.pad = 0;
+      r.pad = Tfin;
       r.cells = cells;
       r.slots = slots;
       A.res[pid] = r;

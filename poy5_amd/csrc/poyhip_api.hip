@@ -184,6 +184,7 @@ int run_linear_one(poyhip_ctx* c, const uint8_t* s1, int l1, const uint8_t* s2, 
   A.stab = static_cast<long long*>(c->stab.p);
   A.res = dres;
   A.counter = counter;
+  A.phase = 0;
   hipLaunchKernelGGL(k_linear_fill<true>, dim3(1), dim3(64), 0, c->stream, A);
   HIPCHK(c, hipGetLastError());
   size_t back = sizeof(PairResult);
@@ -569,6 +570,11 @@ struct poyhip_batch {
   // timing
   std::vector<hipEvent_t> ev;
   float fill_ms = 0.f, total_ms = 0.f;
+  // two-phase plan (long linear pairs): cost-only rounds for every pair,
+  // then direction stores sized to each pair's final band
+  bool two_phase = false;
+  int64_t dm_budget = 0;
+  std::vector<PairResult> h_res;
   // graph replay
   bool use_graph = false;
   hipGraphExec_t gexec = nullptr;
@@ -594,6 +600,21 @@ int64_t ws_ints(int op, int lb) {
   if (op == POYHIP_OP_ALIGN_AFFINE) return aff_ws_ints(lb);
   if (op == POYHIP_OP_COST_AFFINE) return affcost_ws_ints(lb);
   return lin_ws_ints(lb);
+}
+
+// Direction bytes of the final round of a linear pair with band half-width
+// newk (the stripe loop of k_linear_fill).
+int64_t lin_dm_exact(int L1, int L2, int newk, bool full) {
+  const int Dd = L2 - L1;
+  int64_t tot = 0;
+  for (int i0 = 1; i0 <= L1 - 1; i0 += 64) {
+    const int ilast = std::min(i0 + 63, L1 - 1);
+    const int jlo = full ? 0 : std::max(i0 - newk, 0);
+    const int jhi = full ? L2 - 1 : std::min(ilast + Dd + newk, L2 - 1);
+    const int nsteps = (jhi - jlo + 1) + (ilast - i0);
+    tot += (int64_t)((nsteps + 15) >> 4) * 256;
+  }
+  return tot;
 }
 
 void batch_free_device(poyhip_batch* b) {
@@ -695,6 +716,7 @@ int batch_enqueue(poyhip_batch* b, const uint8_t* blob, const int64_t* off1, con
         A.stab = b->d_stab;
         A.res = b->d_res;
         A.counter = b->d_counters + k;
+        A.phase = 0;
         if (b->op == POYHIP_OP_ALIGN_2D)
           hipLaunchKernelGGL(k_linear_fill<true>, dim3(b->grid), dim3(64), 0, st, A);
         else
@@ -793,9 +815,17 @@ int poyhip_batch_create(poyhip_ctx* c, int op, int n, const int* len1, const int
     if (hipMemGetInfo(&fr, &tot) != hipSuccess) fr = (size_t)8 << 30;
     dm_budget = std::min(fr / 2, (size_t)96 << 30);
   }
+  b->dm_budget = (int64_t)dm_budget;
+  if (op == POYHIP_OP_ALIGN_2D) {
+    int64_t est = 0;
+    for (int k = 0; k < n; k++) est += (pair_dm_bytes(op, b->la[k], b->lb[k]) + 255) & ~(int64_t)255;
+    // direction stores sized for the full width would need several chunks
+    // (few pairs in flight): plan per run from the pairs' final bands
+    b->two_phase = est > (int64_t)dm_budget;
+  }
   int64_t cur = 0;
   int begin = 0;
-  for (int q = 0; q < n; q++) {
+  for (int q = 0; q < n && !b->two_phase; q++) {
     const int k = b->order[q];
     const int64_t sz = (pair_dm_bytes(op, b->la[k], b->lb[k]) + 255) & ~(int64_t)255;
     if (cur > 0 && cur + sz > (int64_t)dm_budget) {
@@ -806,7 +836,12 @@ int poyhip_batch_create(poyhip_ctx* c, int op, int n, const int* len1, const int
     b->dm_off[k] = cur;
     cur += sz;
   }
-  b->chunks.push_back({begin, n - begin, cur});
+  if (b->two_phase) {
+    b->chunks.push_back({0, n, 0});
+    b->dm_cap = b->dm_budget;
+  } else {
+    b->chunks.push_back({begin, n - begin, cur});
+  }
   for (const auto& ch : b->chunks) b->dm_cap = std::max(b->dm_cap, ch.dm_bytes);
   // persistent grid: waves resident across the device
   b->grid = std::min(n, c->num_cus * 16);
@@ -833,7 +868,8 @@ int poyhip_batch_create(poyhip_ctx* c, int op, int n, const int* len1, const int
   b->d_order = reinterpret_cast<int*>(p); p += 4 * (size_t)n;
   if (hipMalloc(&b->d_res, sizeof(PairResult) * (size_t)n) != hipSuccess)
     return fail(POYHIP_ENOMEM, "results");
-  if (hipMalloc(&b->d_counters, sizeof(int) * (2 * b->chunks.size() + 2)) != hipSuccess)
+  const size_t ncounters = b->two_phase ? 2 * (size_t)n + 4 : 2 * b->chunks.size() + 2;
+  if (hipMalloc(&b->d_counters, sizeof(int) * ncounters) != hipSuccess)
     return fail(POYHIP_ENOMEM, "counters");
   if (hipMalloc(&b->d_cls, (size_t)n + 64) != hipSuccess) return fail(POYHIP_ENOMEM, "classes");
   if (op_has_dm(op)) {
@@ -902,6 +938,111 @@ int poyhip_batch_use_graph(poyhip_batch* b, int enable) {
   return POYHIP_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Two-phase run of a linear align batch: phase 1 runs every doubling round
+// of every pair without directions (all pairs in flight, no direction
+// memory), phase 2 repeats each pair's final round with directions, in
+// chunks planned from the final bands.  Synchronous.
+int batch_run_two_phase(poyhip_batch* b, const uint8_t* d_blob, const int64_t* d_off1,
+                        const int64_t* d_off2, int* d_cost, uint8_t* d_out, int* d_outlen) {
+  poyhip_ctx* c = b->ctx;
+  hipStream_t st = c->stream;
+  const int n = b->n;
+  std::vector<hipEvent_t> ev(4);
+  for (auto& e : ev) HIPCHK(c, hipEventCreate(&e));
+  struct EvGuard {
+    std::vector<hipEvent_t>* v;
+    ~EvGuard() { for (auto e : *v) (void)hipEventDestroy(e); }
+  } guard{&ev};
+  HIPCHK(c, hipEventRecord(ev[0], st));
+  hipLaunchKernelGGL(k_zero_ints, dim3((2 * n + 4 + 255) / 256), dim3(256), 0, st, b->d_counters,
+                     2 * n + 4);
+  PairTable all = batch_table(b, d_blob, d_off1, d_off2, -1);
+  hipLaunchKernelGGL(k_linear_deltawh, dim3((n + 255) / 256), dim3(256), 0, st, all, b->d_dw,
+                     b->has_deltaw, b->d_dwin);
+  LinFillArgs A;
+  A.pt = all;
+  A.cm = c->d_cm;
+  A.dm = b->d_dm;
+  A.ws = b->d_ws;
+  A.ws_stride = b->ws_stride;
+  A.stab = b->d_stab;
+  A.res = b->d_res;
+  A.counter = b->d_counters;
+  A.phase = 1;
+  HIPCHK(c, hipEventRecord(ev[1], st));
+  hipLaunchKernelGGL(k_linear_fill<false>, dim3(b->grid), dim3(64), 0, st, A);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipEventRecord(ev[2], st));
+  b->h_res.resize(n);
+  HIPCHK(c, hipMemcpyAsync(b->h_res.data(), b->d_res, sizeof(PairResult) * (size_t)n,
+                           hipMemcpyDeviceToHost, st));
+  HIPCHK(c, hipStreamSynchronize(st));
+  float ms1 = 0.f;
+  (void)hipEventElapsedTime(&ms1, ev[1], ev[2]);
+  // chunks of the final round, largest pairs first
+  b->chunks.clear();
+  int64_t cur = 0;
+  int begin = 0;
+  for (int q = 0; q < n; q++) {
+    const int k = b->order[q];
+    const PairResult& r = b->h_res[k];
+    const int64_t sz =
+        (lin_dm_exact(b->la[k], b->lb[k], r.newk < 0 ? 0 : r.newk, r.full != 0) + 255) & ~(int64_t)255;
+    if (sz > b->dm_budget)
+      return set_err(c, POYHIP_ECAP, "pair %d needs %lld direction bytes", k, (long long)sz);
+    if (cur > 0 && cur + sz > b->dm_budget) {
+      b->chunks.push_back({begin, q - begin, cur});
+      begin = q;
+      cur = 0;
+    }
+    b->dm_off[k] = cur;
+    cur += sz;
+  }
+  b->chunks.push_back({begin, n - begin, cur});
+  HIPCHK(c, hipMemcpyAsync(b->d_dm_off, b->dm_off.data(), 8 * (size_t)n, hipMemcpyHostToDevice, st));
+  float ms2 = 0.f;
+  for (size_t k = 0; k < b->chunks.size(); k++) {
+    PairTable t = batch_table(b, d_blob, d_off1, d_off2, (int)k);
+    A.pt = t;
+    A.counter = b->d_counters + 1 + k;
+    A.phase = 2;
+    HIPCHK(c, hipEventRecord(ev[1], st));
+    hipLaunchKernelGGL(k_linear_fill<true>, dim3(std::min(b->grid, t.n)), dim3(64), 0, st, A);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventRecord(ev[2], st));
+    LinBtArgs B;
+    B.pt = t;
+    B.cm = c->d_cm;
+    B.dm = b->d_dm;
+    B.stab = b->d_stab;
+    B.res = b->d_res;
+    B.out = d_out;
+    B.outlen = d_outlen;
+    hipLaunchKernelGGL(k_linear_backtrace, dim3((t.n + 63) / 64), dim3(64), 0, st, B);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventSynchronize(ev[2]));
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, ev[1], ev[2]);
+    ms2 += ms;
+  }
+  hipLaunchKernelGGL(k_extract_cost, dim3((n + 255) / 256), dim3(256), 0, st, b->d_res, d_cost, n);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipEventRecord(ev[3], st));
+  HIPCHK(c, hipStreamSynchronize(st));
+  b->fill_ms = ms1 + ms2;
+  float all_ms = 0.f;
+  if (hipEventElapsedTime(&all_ms, ev[0], ev[3]) == hipSuccess) b->total_ms = all_ms;
+  return POYHIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int poyhip_batch_run(poyhip_batch* b, const uint8_t* d_blob, const int64_t* d_off1,
                      const int64_t* d_off2, int* d_cost, uint8_t* d_out, int* d_outlen, int sync) {
   if (!b || !d_blob || !d_off1 || !d_off2 || !d_cost) return POYHIP_EINVAL;
@@ -910,6 +1051,7 @@ int poyhip_batch_run(poyhip_batch* b, const uint8_t* d_blob, const int64_t* d_of
     return set_err(c, POYHIP_EINVAL, "align batches need output buffers");
   (void)hipSetDevice(c->device);
   hipStream_t st = c->stream;
+  if (b->two_phase) return batch_run_two_phase(b, d_blob, d_off1, d_off2, d_cost, d_out, d_outlen);
   if (b->use_graph) {
     const void* key[6] = {d_blob, d_off1, d_off2, d_cost, d_out, d_outlen};
     if (b->gexec && memcmp(key, b->g_key, sizeof key) != 0) {

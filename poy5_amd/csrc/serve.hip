@@ -326,6 +326,7 @@ int srv_launch(poyhip_server* s, SrvSlot& sl) {
     A.stab = sl.d_stab;
     A.res = sl.d_res;
     A.counter = sl.d_counters;
+    A.phase = 0;
     if (op == POYHIP_OP_ALIGN_2D) {
       hipLaunchKernelGGL(k_linear_fill<true>, dim3(grid), dim3(64), 0, st, A);
       LinBtArgs B;

@@ -182,3 +182,38 @@ def test_batch_linear_config1_sample(ctx):
         assert np.array_equal(outs[k][3], oracle.median_2_with_gaps(o1, o2, ocm)), k
     costs2, _ = run_pairs(ctx, "cost_2d", pairs)
     assert np.array_equal(costs2, costs)
+
+
+def test_batch_linear_config4_long(ctx):
+    """Long banded pairs of the config-4 process (16k+ symbols, 10% IUPAC,
+    sub 1 / indel 1): the two shortest of 12 draws, bit-exact."""
+    say("test_batch_linear_config4_long")
+    ctx.set_cm(cost_matrix.of_transformations_and_gaps(1, 1))
+    ocm = oracle.cm_dna(1, 1)
+    pairs = workloads.pairs_list(4, 12)
+    pairs.sort(key=lambda p: len(p[0]) * len(p[1]))
+    pairs = pairs[:2]
+    costs, outs = run_pairs(ctx, "align_2d", pairs)
+    for k, (a, b) in enumerate(pairs):
+        o1, o2, oc, st = oracle.align_2(a, b, ocm)
+        assert not st.full_plane
+        assert costs[k] == oc, (k, costs[k], oc)
+        assert np.array_equal(outs[k][0], o1) and np.array_equal(outs[k][1], o2), k
+
+
+def test_batch_linear_two_phase(ctx):
+    """A direction-store budget far below the full-width estimate: cost-only
+    rounds for every pair, then final rounds in chunks sized to the bands.
+    Same results as the one-phase plan and the oracle."""
+    say("test_batch_linear_two_phase")
+    ctx.set_cm(cost_matrix.of_transformations_and_gaps(1, 2))
+    ocm = oracle.cm_dna(1, 2)
+    pairs = workloads.pairs_list(1, 48)
+    c1, o1 = run_pairs(ctx, "align_2d", pairs)
+    c2, o2 = run_pairs(ctx, "align_2d", pairs, dm_budget=1 << 20)
+    assert np.array_equal(c1, c2)
+    for k, (a, b) in enumerate(pairs):
+        for r in range(4):
+            assert np.array_equal(o1[k][r], o2[k][r]), (k, r)
+        r1, r2, oc, _ = oracle.align_2(a, b, ocm)
+        assert c2[k] == oc and np.array_equal(o2[k][0], r1) and np.array_equal(o2[k][1], r2), k
