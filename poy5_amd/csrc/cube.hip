@@ -93,10 +93,230 @@ __device__ __forceinline__ void store16(int* p, const int v[kCubeCpl]) {
   for (int u = 0; u < 4; u++) q[u] = make_int4(v[4 * u], v[4 * u + 1], v[4 * u + 2], v[4 * u + 3]);
 }
 
+// One step of the min-plus map scan: the maps (pa, pb) of the lanes before
+// are applied first, then the lane's own (fa, fb):
+//   y -> min(fa, min(pa, y + pb) + fb) = min(min(fa, pa + fb), y + pb + fb).
+// Lanes without a source keep the identity map (kCubeInf, 0).
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ void cube_scan_step(int& fa, int& fb) {
+  const int pa = __builtin_amdgcn_update_dpp(kCubeInf, fa, kCtrl, kRowMask, 0xF, false);
+  const int pb = __builtin_amdgcn_update_dpp(0, fb, kCtrl, kRowMask, 0xF, false);
+  fa = imin(fa, pa + fb);
+  fb = imin(pb + fb, kCubeInf);
+}
+
+// Inclusive scan of the maps over the 64 lanes.
+__device__ __forceinline__ void cube_scan_minplus(int& fa, int& fb) {
+  cube_scan_step<0x111, 0xF>(fa, fb);  // row_shr:1
+  cube_scan_step<0x112, 0xF>(fa, fb);  // row_shr:2
+  cube_scan_step<0x114, 0xF>(fa, fb);  // row_shr:4
+  cube_scan_step<0x118, 0xF>(fa, fb);  // row_shr:8
+  cube_scan_step<0x142, 0xA>(fa, fb);  // row_bcast:15 into rows 1 and 3
+  cube_scan_step<0x143, 0xC>(fa, fb);  // row_bcast:31 into rows 2 and 3
+}
+
 // Row kinds of algn_fill_cube.
 enum : int { kRowFirst = 0, kRowPlane0 = 1, kRowPlaneStart = 2, kRowInner = 3 };
 
+// Cells of one row segment of algn_fill_cube: the six plane/line moves in
+// the reference's order, then the SS run along the row as a min-plus scan.
+// cU/cP/cD are the k-1 inputs of lane 0's first cell (previous segment),
+// cY the final value before it.  Returns the row values in X, the direction
+// nibbles in lo/hi and the lane's last value in y.
+__device__ __forceinline__ void cube_row(int kind, int k0, const int cc[kCubeCpl],
+                                         const int GG[kCubeCpl], const int U[kCubeCpl],
+                                         const int P[kCubeCpl], const int D[kCubeCpl], int cU,
+                                         int cP, int cD, int cY, int s1g0, int g0, int s120,
+                                         const int* s_s1g, const int* s_gs2, const int* s_s12,
+                                         int X[kCubeCpl], uint32_t& lo, uint32_t& hi, int& y) {
+  const int lane = threadIdx.x;
+  int Dir[kCubeCpl];
+  const int nU = wave_shr1(U[kCubeCpl - 1], cU);
+  const int nP = wave_shr1(P[kCubeCpl - 1], cP);
+  const int nD = wave_shr1(D[kCubeCpl - 1], cD);
+#pragma unroll
+  for (int m = 0; m < kCubeCpl; m++) {
+    const int k = k0 + m;
+    const int c = cc[m];
+    const int um1 = m == 0 ? nU : U[m - 1];
+    const int pm1 = m == 0 ? nP : P[m - 1];
+    const int dm1 = m == 0 ? nD : D[m - 1];
+    int v, d;
+    if (kind == kRowFirst) {
+      v = k == 0 ? 0 : kCubeInf;
+      d = k == 0 ? kD3S2 : kD3SS;
+    } else if (kind == kRowPlane0) {
+      v = P[m] + g0;
+      d = kD3P1;
+      const int t1 = pm1 + s_gs2[c];
+      if (k >= 1 && t1 < v) { v = t1; d = kD3S1; }
+    } else if (kind == kRowPlaneStart) {
+      v = D[m] + s1g0;
+      d = kD3P3;
+      const int t1 = dm1 + s_s1g[c];
+      if (k >= 1 && t1 < v) { v = t1; d = kD3S3; }
+    } else {
+      v = U[m] + s1g0;
+      d = kD3P3;
+      int t1 = P[m] + g0;
+      if (t1 < v) { v = t1; d = kD3P1; }
+      t1 = D[m] + s120;
+      if (t1 < v) { v = t1; d = kD3P2; }
+      if (k >= 1) {
+        t1 = um1 + s_s1g[c];
+        if (t1 < v) { v = t1; d = kD3S3; }
+        t1 = pm1 + s_gs2[c];
+        if (t1 < v) { v = t1; d = kD3S1; }
+        t1 = dm1 + s_s12[c];
+        if (t1 < v) { v = t1; d = kD3S2; }
+      }
+    }
+    X[m] = v;
+    Dir[m] = d;
+  }
+  // SS scan, pass 1: the lane's map y -> min(fa, y + fb)
+  const bool has_k0 = (k0 == 0);
+  int fa = kCubeInf, fb = 0;
+#pragma unroll
+  for (int m = 0; m < kCubeCpl; m++) {
+    if (has_k0 && m == 0) {
+      fa = X[0];
+      fb = kCubeInf;  // the chain starts at k = 0: y is ignored
+    } else {
+      fa = imin(X[m], fa + GG[m]);
+      fb = imin(fb + GG[m], kCubeInf);
+    }
+  }
+  // inclusive scan of the maps over the wavefront (DPP: row_shr inside rows
+  // of 16 lanes, then row_bcast:15 / row_bcast:31)
+  cube_scan_minplus(fa, fb);
+  const int ia = wave_shr1(fa, kCubeInf);
+  const int ib = wave_shr1(fb, 0);
+  y = lane == 0 ? cY : imin(ia, cY + ib);
+  // pass 2: apply the incoming value
+  lo = 0;
+  hi = 0;
+#pragma unroll
+  for (int m = 0; m < kCubeCpl; m++) {
+    int v = X[m], d = Dir[m];
+    if (!(has_k0 && m == 0)) {
+      const int t1 = y + GG[m];
+      if (t1 < v) { v = t1; d = kD3SS; }
+    }
+    X[m] = v;
+    y = v;
+    if (m < 8) lo |= (uint32_t)d << (4 * m);
+    else hi |= (uint32_t)d << (4 * (m - 8));
+  }
+}
+
+// Triples whose rows fit one segment (L3 <= 1024).  Besides the ring, the
+// row pointers give registers most inputs: a plane-0 row reads the row just
+// computed, and an inner row's diag row is the previous row's upper row (the
+// plane-start row's input for row 1).  The two rows an inner row still
+// loads, and its precalc entries, are fetched one row ahead: row R+1 of
+// plane i >= 1 reads rows at least i+1 behind it, all stored before row R.
 template <bool kDm>
+__device__ int cube_fast(const CubeFillArgs& A, const uint8_t* s1, const uint8_t* s2, int L1,
+                         int L2, int L3, int g3, int* ring, uint8_t* dm, const int cc[kCubeCpl],
+                         const int GG[kCubeCpl], int* s_s1g, int* s_gs2, int* s_s12) {
+  const int lane = threadIdx.x;
+  const int k0 = lane * kCubeCpl;
+  const int NR = L1 + L2;
+  auto row_ptr = [&](int r) { return ring + (long long)(r % NR) * kCubeSeg + k0; };
+  int U[kCubeCpl], P[kCubeCpl], D[kCubeCpl], nU[kCubeCpl], nP[kCubeCpl];
+#pragma unroll
+  for (int m = 0; m < kCubeCpl; m++) U[m] = P[m] = D[m] = nU[m] = nP[m] = 0;
+  // precalc entries of the next row (lanes 0..31 hold entry `lane`)
+  int tG = 0, tA = 0, nG0 = 0, nS120 = 0;
+  auto fetch_tables = [&](int i, int j) {
+    const int a = s1[i] & 31, b = s2[j] & 31;
+    if (j >= 1) {
+      tG = cube_c3(A.cm3, kGap, b, lane & 31);
+      nG0 = cube_c3(A.cm3, kGap, b, g3);
+      if (i >= 1) {
+        tA = cube_c3(A.cm3, a, b, lane & 31);
+        nS120 = cube_c3(A.cm3, a, b, g3);
+      }
+    }
+  };
+  fetch_tables(0, L2 > 1 ? 1 : 0);
+  int s1g0 = 0, last = 0;
+  for (int i = 0; i < L1; i++) {
+    if (i >= 1) {
+      const int a = s1[i] & 31;
+      __builtin_amdgcn_wave_barrier();
+      if (lane < 32) s_s1g[lane] = cube_c3(A.cm3, a, kGap, lane);
+      s1g0 = cube_c3(A.cm3, a, kGap, g3);
+    }
+    for (int j = 0; j < L2; j++) {
+      const int R = i * L2 + j;
+      const int kind = i == 0 ? (j == 0 ? kRowFirst : kRowPlane0)
+                              : (j == 0 ? kRowPlaneStart : kRowInner);
+      int g0 = 0, s120 = 0;
+      if (j >= 1) {
+        // this row's precalc entries, fetched during the previous row
+        __builtin_amdgcn_wave_barrier();
+        if (lane < 32) {
+          s_gs2[lane] = tG;
+          if (i >= 1) s_s12[lane] = tA;
+        }
+        g0 = nG0;
+        s120 = nS120;
+      }
+      __builtin_amdgcn_wave_barrier();
+      if (kind == kRowPlaneStart) {
+        load16(row_ptr((i - 1) * (L2 - 1)), D);
+      } else if (kind == kRowInner) {
+#pragma unroll
+        for (int m = 0; m < kCubeCpl; m++) {
+          U[m] = nU[m];
+          P[m] = nP[m];
+        }
+      }
+      // fetch the next row's inputs and precalc entries
+      const int Rn = R + 1;
+      if (Rn < L1 * L2) {
+        const int in = Rn / L2, jn = Rn - in * L2;
+        if (in >= 1 && jn >= 1) {
+          load16(row_ptr((in - 1) * (L2 - 1) + jn), nU);
+          load16(row_ptr(in * (L2 - 1) + jn - 1), nP);
+        }
+        if (jn >= 1) fetch_tables(in, jn);
+      }
+      int X[kCubeCpl];
+      uint32_t lo, hi;
+      int y;
+      cube_row(kind, k0, cc, GG, U, P, D, 0, 0, 0, kCubeInf, s1g0, g0, s120, s_s1g, s_gs2, s_s12,
+               X, lo, hi, y);
+      store16(row_ptr(R), X);
+      if (kDm) reinterpret_cast<uint2*>(dm + (long long)R * 512)[lane] = make_uint2(lo, hi);
+      // register chains for the next row
+      if (kind == kRowInner) {
+#pragma unroll
+        for (int m = 0; m < kCubeCpl; m++) D[m] = U[m];
+      }
+      if (i == 0) {
+#pragma unroll
+        for (int m = 0; m < kCubeCpl; m++) P[m] = X[m];
+      }
+      if (R == L1 * L2 - 1) {
+        // the last cell, k = L3 - 1, belongs to lane (L3 - 1) / 16
+        const int mm = (L3 - 1) % kCubeCpl;
+        int sel = 0;
+#pragma unroll
+        for (int m = 0; m < kCubeCpl; m++) sel = (m == mm) ? X[m] : sel;
+        last = __shfl(sel, (L3 - 1) / kCubeCpl);
+      }
+    }
+  }
+  return last;
+}
+
+// kOneSeg: the launch takes only triples with L3 <= 1024 (cube_fast); the
+// other instantiation takes only the longer ones.  Each instantiation gets
+// the registers of its own path.
+template <bool kDm, bool kOneSeg>
 __global__ __launch_bounds__(64) void k_cube_fill(CubeFillArgs A) {
   // precalc rows of the current plane / row indexed by the s3 code
   // (cm_precalc_4algn_3d, src/cm.c:1390-1412)
@@ -117,6 +337,7 @@ __global__ __launch_bounds__(64) void k_cube_fill(CubeFillArgs A) {
     const uint8_t* s3 = A.tt.blob + A.tt.off3[t];
     const int L1 = A.tt.len1[t], L2 = A.tt.len2[t], L3 = A.tt.len3[t];
     const int nseg = cube_nseg(L3);
+    if ((nseg == 1) != kOneSeg) continue;
     const int row_ints = nseg * kCubeSeg;
     const int NR = L1 + L2;
     int* ring = A.ring + (long long)blockIdx.x * A.ring_stride;
@@ -136,7 +357,13 @@ __global__ __launch_bounds__(64) void k_cube_fill(CubeFillArgs A) {
         GG[m] = s_gg[cc[m]];
       }
     };
-    if (nseg == 1) load_codes(0);
+    if constexpr (kOneSeg) {
+      load_codes(0);
+      const int lv =
+          cube_fast<kDm>(A, s1, s2, L1, L2, L3, g3, ring, dm, cc, GG, s_s1g, s_gs2, s_s12);
+      if (lane == 0) A.cost[t] = lv;
+      continue;
+    }
     int last_val = 0;
     for (int i = 0; i < L1; i++) {
       const int a = s1[i] & 31;
@@ -190,90 +417,11 @@ __global__ __launch_bounds__(64) void k_cube_fill(CubeFillArgs A) {
           } else if (kind == kRowPlaneStart) {
             load16(pD + k0, D);
           }
-          int X[kCubeCpl], Dir[kCubeCpl];
-          const int nU = wave_shr1(U[kCubeCpl - 1], cU);
-          const int nP = wave_shr1(P[kCubeCpl - 1], cP);
-          const int nD = wave_shr1(D[kCubeCpl - 1], cD);
-#pragma unroll
-          for (int m = 0; m < kCubeCpl; m++) {
-            const int k = k0 + m;
-            const int c = cc[m];
-            const int um1 = m == 0 ? nU : U[m - 1];
-            const int pm1 = m == 0 ? nP : P[m - 1];
-            const int dm1 = m == 0 ? nD : D[m - 1];
-            int v, d;
-            if (kind == kRowFirst) {
-              v = k == 0 ? 0 : kCubeInf;
-              d = k == 0 ? kD3S2 : kD3SS;
-            } else if (kind == kRowPlane0) {
-              v = P[m] + g0;
-              d = kD3P1;
-              const int t1 = pm1 + s_gs2[c];
-              if (k >= 1 && t1 < v) { v = t1; d = kD3S1; }
-            } else if (kind == kRowPlaneStart) {
-              v = D[m] + s1g0;
-              d = kD3P3;
-              const int t1 = dm1 + s_s1g[c];
-              if (k >= 1 && t1 < v) { v = t1; d = kD3S3; }
-            } else {
-              v = U[m] + s1g0;
-              d = kD3P3;
-              int t1 = P[m] + g0;
-              if (t1 < v) { v = t1; d = kD3P1; }
-              t1 = D[m] + s120;
-              if (t1 < v) { v = t1; d = kD3P2; }
-              if (k >= 1) {
-                t1 = um1 + s_s1g[c];
-                if (t1 < v) { v = t1; d = kD3S3; }
-                t1 = pm1 + s_gs2[c];
-                if (t1 < v) { v = t1; d = kD3S1; }
-                t1 = dm1 + s_s12[c];
-                if (t1 < v) { v = t1; d = kD3S2; }
-              }
-            }
-            X[m] = v;
-            Dir[m] = d;
-          }
-          // SS scan, pass 1: the lane's map y -> min(fa, y + fb)
-          const bool has_k0 = (k0 == 0);
-          int fa = kCubeInf, fb = 0;
-#pragma unroll
-          for (int m = 0; m < kCubeCpl; m++) {
-            if (has_k0 && m == 0) {
-              fa = X[0];
-              fb = kCubeInf;  // the chain starts at k = 0: y is ignored
-            } else {
-              fa = imin(X[m], fa + GG[m]);
-              fb = imin(fb + GG[m], kCubeInf);
-            }
-          }
-          // inclusive scan of the maps over the wavefront
-#pragma unroll
-          for (int dd = 1; dd < 64; dd <<= 1) {
-            const int pa = __shfl(fa, lane - dd < 0 ? 0 : lane - dd);
-            const int pb = __shfl(fb, lane - dd < 0 ? 0 : lane - dd);
-            if (lane >= dd) {
-              fa = imin(fa, pa + fb);
-              fb = imin(pb + fb, kCubeInf);
-            }
-          }
-          const int ia = __shfl(fa, lane == 0 ? 0 : lane - 1);
-          const int ib = __shfl(fb, lane == 0 ? 0 : lane - 1);
-          int y = lane == 0 ? cY : imin(ia, cY + ib);
-          // pass 2: apply the incoming value
-          uint32_t lo = 0, hi = 0;
-#pragma unroll
-          for (int m = 0; m < kCubeCpl; m++) {
-            int v = X[m], d = Dir[m];
-            if (!(has_k0 && m == 0)) {
-              const int t1 = y + GG[m];
-              if (t1 < v) { v = t1; d = kD3SS; }
-            }
-            X[m] = v;
-            y = v;
-            if (m < 8) lo |= (uint32_t)d << (4 * m);
-            else hi |= (uint32_t)d << (4 * (m - 8));
-          }
+          int X[kCubeCpl];
+          uint32_t lo, hi;
+          int y;
+          cube_row(kind, k0, cc, GG, U, P, D, cU, cP, cD, cY, s1g0, g0, s120, s_s1g, s_gs2, s_s12,
+                   X, lo, hi, y);
           store16(pC + k0, X);
           if (kDm) {
             uint2* w = reinterpret_cast<uint2*>(dm + (long long)R * (nseg * 512) + sg * 512);

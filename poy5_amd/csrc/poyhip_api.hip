@@ -1190,7 +1190,10 @@ int run_cube_one(poyhip_ctx* c, const uint8_t* s1, int l1, const uint8_t* s2, in
   A.ring_stride = 0;
   A.cost = dcost;
   A.counter = &reinterpret_cast<OneDesc3*>(c->stage.p)->counter;
-  hipLaunchKernelGGL(k_cube_fill<true>, dim3(1), dim3(64), 0, c->stream, A);
+  if (cube_nseg(l3) == 1)
+    hipLaunchKernelGGL((k_cube_fill<true, true>), dim3(1), dim3(64), 0, c->stream, A);
+  else
+    hipLaunchKernelGGL((k_cube_fill<true, false>), dim3(1), dim3(64), 0, c->stream, A);
   HIPCHK(c, hipGetLastError());
   size_t back = 4 * sizeof(int);
   if (backtrack) {
@@ -1466,7 +1469,7 @@ int poyhip_batch3_create(poyhip_ctx* c, int op, int n, const int* len1, const in
   b->d_l2 = reinterpret_cast<int*>(p); p += 4 * (size_t)n;
   b->d_l3 = reinterpret_cast<int*>(p); p += 4 * (size_t)n;
   b->d_order = reinterpret_cast<int*>(p); p += 4 * (size_t)n;
-  if (hipMalloc(&b->d_counters, sizeof(int) * (b->chunks.size() + 2)) != hipSuccess)
+  if (hipMalloc(&b->d_counters, sizeof(int) * (2 * b->chunks.size() + 2)) != hipSuccess)
     return fail(POYHIP_ENOMEM, "counters");
   if (op == POYHIP_OP_ALIGN_3D && hipMalloc(&b->d_dm, (size_t)b->dm_cap + 256) != hipSuccess)
     return fail(POYHIP_ENOMEM, "direction store");
@@ -1525,7 +1528,7 @@ int poyhip_batch3_run(poyhip_batch3* b, const uint8_t* d_blob, const int64_t* d_
   hipStream_t st = c->stream;
   const int nch = (int)b->chunks.size();
   HIPCHK(c, hipEventRecord(b->ev[2 * nch], st));
-  hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, st, b->d_counters, nch);
+  hipLaunchKernelGGL(k_zero_ints, dim3((2 * nch + 63) / 64), dim3(64), 0, st, b->d_counters, 2 * nch);
   for (int k = 0; k < nch; k++) {
     const auto& ch = b->chunks[k];
     TripleTable t;
@@ -1551,10 +1554,17 @@ int poyhip_batch3_run(poyhip_batch3* b, const uint8_t* d_blob, const int64_t* d_
     A.counter = b->d_counters + k;
     const int g = std::min(b->grid, ch.count);
     HIPCHK(c, hipEventRecord(b->ev[2 * k], st));
+    // one launch per row shape (counters 2k and 2k+1)
+    A.counter = b->d_counters + 2 * k;
     if (b->op == POYHIP_OP_ALIGN_3D)
-      hipLaunchKernelGGL(k_cube_fill<true>, dim3(g), dim3(64), 0, st, A);
+      hipLaunchKernelGGL((k_cube_fill<true, true>), dim3(g), dim3(64), 0, st, A);
     else
-      hipLaunchKernelGGL(k_cube_fill<false>, dim3(g), dim3(64), 0, st, A);
+      hipLaunchKernelGGL((k_cube_fill<false, true>), dim3(g), dim3(64), 0, st, A);
+    A.counter = b->d_counters + 2 * k + 1;
+    if (b->op == POYHIP_OP_ALIGN_3D)
+      hipLaunchKernelGGL((k_cube_fill<true, false>), dim3(g), dim3(64), 0, st, A);
+    else
+      hipLaunchKernelGGL((k_cube_fill<false, false>), dim3(g), dim3(64), 0, st, A);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipEventRecord(b->ev[2 * k + 1], st));
     if (b->op == POYHIP_OP_ALIGN_3D) {
