@@ -122,19 +122,18 @@ __device__ __forceinline__ void aff_cell(const AffRow& r, int go, int sw, int j,
 // state stays >= HIGH_NUM so it never reaches a minimum or a tie, and its
 // END_BLOCK flag is set on every column past 0.  What remains is Gotoh's
 // recurrence with POY's tie flags.
-__device__ __forceinline__ void aff_cell_fast(const AffRow& r, int go, int sw, int j, int jc,
-                                              bool at_left, bool at_right, const int* s_cost,
-                                              const int* s_prep, int uV, int uCB, int uG, int dV,
-                                              int dCB, int dH, int dG, int lH, int lCB, int lG,
-                                              AffCell& o) {
-  const int gr = s_prep[jc];
+// The column's table entries come in: gr = prepend[jc] (horizontal gap
+// cost of the column symbol) and d = cost(row symbol, jc).
+__device__ __forceinline__ void aff_cell_fast_core(int ge, int go, int sw, int j, int gr, int d,
+                                                   bool at_left, bool at_right, int uV, int uCB,
+                                                   int uG, int dV, int dCB, int dH, int dG, int lH,
+                                                   int lCB, int lG, AffCell& o) {
   const int hb = lCB + go;
   const int H = at_left ? kHigh : gr + imin(lH, hb);
   const int endH = at_left ? 0 : (lH >= hb);
   const int vb = uCB + go;
-  const int V = at_right ? kHigh : r.ge + imin(uV, vb);
+  const int V = at_right ? kHigh : ge + imin(uV, vb);
   const int endV = at_right ? 0 : (uV >= vb);
-  const int d = s_cost[r.cost_row + (jc & 15)];
   const int m3 = imin(imin(dCB, dV), dH);
   const bool j0 = (j == 0);
   const int C = j0 ? kHigh : m3 + d;
@@ -158,6 +157,16 @@ __device__ __forceinline__ void aff_cell_fast(const AffRow& r, int go, int sw, i
   o.H = H;
   o.E = kHigh;
   o.f = f;
+}
+
+
+__device__ __forceinline__ void aff_cell_fast(const AffRow& r, int go, int sw, int j, int jc,
+                                              bool at_left, bool at_right, const int* s_cost,
+                                              const int* s_prep, int uV, int uCB, int uG, int dV,
+                                              int dCB, int dH, int dG, int lH, int lCB, int lG,
+                                              AffCell& o) {
+  aff_cell_fast_core(r.ge, go, sw, j, s_prep[jc], s_cost[r.cost_row + (jc & 15)], at_left,
+                     at_right, uV, uCB, uG, dV, dCB, dH, dG, lH, lCB, lG, o);
 }
 
 // 1 when a pair can take aff_cell_fast: no symbol past the leading gap of
@@ -337,13 +346,19 @@ __device__ void rot_round_fast(RotShared& sh, int total, const uint8_t* __restri
     // lane 63's FIFO writes of the previous block precede these reads in
     // program order; the barrier keeps the compiler from moving them
     __builtin_amdgcn_wave_barrier();
-    int c0b[4], clb[4];
+    int c0b[4], clb[4], grb[4], cdb[4];
     int4 fv[4];
+    // table entries of the block's columns, looked up together: the lane's
+    // row symbol changes at its stripe switch (step nstart)
+    const int icn = ((bt + 1) & 1) ? symB : symA;
+    const int crn = (icn & 15) << 5;
 #pragma unroll
     for (int u = 0; u < 4; u++) {
       c0b[u] = cn[u];
       clb[u] = cl[u];
       fv[u] = sh.fifo[col0(s + u) & (kRotFifo - 1)];
+      grb[u] = s_prep[clb[u]];
+      cdb[u] = s_cost[((s + u) >= nstart ? crn : row.cost_row) + (clb[u] & 15)];
     }
 #pragma unroll
     for (int u = 0; u < 4; u++) {
@@ -410,15 +425,14 @@ __device__ void rot_round_fast(RotShared& sh, int total, const uint8_t* __restri
         sV = fv[u].x; sCB = fv[u].y; sH = fv[u].z; sG = fv[u].w;
       }
       c0prev = c0;
-      const int jc = clb[u];
       const int uV = wave_shr1(oV, sV);
       const int uCB = wave_shr1(oCB, sCB);
       const int uH = wave_shr1(oH, sH);
       const int uG = wave_shr1(oG, sG);
       const int j = st + cbc;
       AffCell o;
-      aff_cell_fast(row, go, sw, j, jc, j == my_lo, j == my_rb, s_cost, s_prep, uV, uCB, uG, dV,
-                    dCB, dH, dG, lH, lCB, lG, o);
+      aff_cell_fast_core(row.ge, go, sw, j, grb[u], cdb[u], j == my_lo, j == my_rb, uV, uCB, uG,
+                         dV, dCB, dH, dG, lH, lCB, lG, o);
       const bool inb = j >= my_lo && j <= my_hi;
       if (inb && lane == 63) sh.fifo[j & (kRotFifo - 1)] = make_int4(o.V, o.C, o.H, o.G);
       if (st == s_fin) {
