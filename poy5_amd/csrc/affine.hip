@@ -223,19 +223,25 @@ __device__ __forceinline__ int band_hi(int i, int D, int newk, int lenj) {
 
 // Plans the round: S_b for every stripe.  Returns the total number of
 // steps, or -1 when the round must use the plain stripe loop.
-//   C1: lane 0 of stripe b+1 reads column j from the FIFO at least 4 steps
-//       after lane 63 of stripe b wrote it (a 4-step block reads its FIFO
-//       entries up front): S_{b+1} >= S_b + 67 + (jlo_{b+1} - jlo_b).
+//   C1: S_{b+1} >= S_b + 71 + (jlo_{b+1} - jlo_b).  Lane t enters stripe b
+//       at S_b + (lo(i0_b + t) - jlo_b) + t <= S_b + (jlo_{b+1} - jlo_b) + 63,
+//       so every lane has entered stripe b at least 8 steps before S_{b+1}:
+//       the entries into stripe b+1 are planned at the start of the 4-step
+//       block before the one holding S_{b+1} (one block ahead, for the column
+//       symbol loads), and lane 0 reads FIFO entries up front per block, at
+//       least 4 steps after lane 63 wrote them.
 //   C2: lane t enters stripe b+1 after its own last cell of stripe b.
 //       Lane t+1 reads lane t's outputs through DPP one step later, so it
 //       still sees lane t's last cell of stripe b; at its own last cell
 //       (the band's right edge) it ignores the row above.
 // The FIFO is safe while lag = S_{b+1} - S_b - (jlo_{b+1} - jlo_b) - 63
 // stays within kRotFifo - 72 (no entry is overwritten before it is read).
+// Gap counts travel as two u16 halves of one word with 32-bit adds, exact
+// while both lengths stay below 2^16 (the counts are bounded by them).
 __device__ int rot_plan(RotShared& sh, int leni, int lenj, int D, int newk, int nst,
                         long long dm_steps_cap) {
   const int lane = threadIdx.x;
-  if (nst > kRotMaxStripes) return -1;
+  if (nst > kRotMaxStripes || lenj > 65535) return -1;
   for (int b = lane; b < nst; b += 64) {
     const int i0 = 1 + (b << 6);
     const int jlo = band_lo(i0, newk);
@@ -246,7 +252,7 @@ __device__ int rot_plan(RotShared& sh, int leni, int lenj, int D, int newk, int 
       for (int t = 0; t < 64 && i0 + t + 64 <= leni; t++)
         c2 = imax(c2, band_hi(i0 + t, D, newk, lenj) - band_lo(i0 + t + 64, newk));
       c2 += (jlon - jlo) + 1;
-      d = imax(67 + (jlon - jlo), c2);
+      d = imax(71 + (jlon - jlo), c2);
       lag = d - (jlon - jlo) - 63;
     } else {
       d = 0;
@@ -275,9 +281,39 @@ __device__ int rot_plan(RotShared& sh, int leni, int lenj, int D, int newk, int 
   return total;
 }
 
+// Byte loads of a pair's column sequence through a buffer resource: offsets
+// past the end (negative ones included, as unsigned) return 0 instead of
+// faulting, so a lane outside its band loads without a clamp (the cells it
+// evaluates there are never read).
+// The pointer and length are wave-uniform; the broadcasts keep the resource
+// in scalar registers.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t seq_rsrc(const uint8_t* p, int n) {
+  const unsigned long long a = reinterpret_cast<unsigned long long>(p);
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)a);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(a >> 32));
+  void* q = reinterpret_cast<void*>(((unsigned long long)hi << 32) | lo);
+  return __builtin_amdgcn_make_buffer_rsrc(q, (short)0, __builtin_amdgcn_readfirstlane(n),
+                                           0x00020000);
+}
+__device__ __forceinline__ int ld_sym(__amdgpu_buffer_rsrc_t r, int off) {
+  return __builtin_amdgcn_raw_buffer_load_b8(r, off, 0, 0);
+}
+
+typedef unsigned short rot_u16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ int max_u16x2(int a, int b) {
+  return __builtin_bit_cast(int, __builtin_elementwise_max(__builtin_bit_cast(rot_u16x2, a),
+                                                            __builtin_bit_cast(rot_u16x2, b)));
+}
+
 // One round of the gap-free fill on the rotating schedule.  Same outputs as
 // the plain stripe loop of k_affine_fill: cost and gap count of the last
 // cell, cells and lane-steps, stripe table and direction words.
+//
+// Every lane keeps its current row in step coordinates (column st + cbc,
+// right edge at step rs, column 0 at step zs) and the entry into its next
+// row as a pending record (start step nstart, same fields), planned for all
+// lanes at once when lane 0 is about to enter a stripe.  A lane switches
+// rows at step nstart, which is also the step of its row's first cell.
 __device__ void rot_round_fast(RotShared& sh, int total, const uint8_t* __restrict__ sa,
                                const uint8_t* __restrict__ sb, int leni, int lenj, int D, int newk,
                                int nst, int go, int sw, int HA0, const int* s_cost,
@@ -286,45 +322,43 @@ __device__ void rot_round_fast(RotShared& sh, int total, const uint8_t* __restri
   const int lane = threadIdx.x;
   const int kNever = 0x7fffffff;
   for (int b = lane; b <= nst; b += 64) stab[b] = sh.S[b];
-  // ---- per-lane state: stripe bt, row i ------------------------------------
-  int bt = 0;
-  int i = 1 + lane;
-  const bool act0 = i <= leni;
+  const __amdgpu_buffer_rsrc_t rsym = seq_rsrc(sb, lenj + 1);
   const int jlo_t = band_lo(1, newk);
-  int my_lo = act0 ? band_lo(i, newk) : (1 << 30);
-  int my_rb = i + D + newk;
-  int my_hi = act0 ? band_hi(i, D, newk, lenj) : -1;
-  if (act0) cells += my_hi - my_lo + 1;
-  // row symbols of stripes b (register b & 1), loaded one stripe ahead when
-  // lane 0 enters stripe b-1
+  // ---- current row (none yet) and pending entry into stripe 0 -------------
+  int cbc = jlo_t - lane, rs = kNever, zs = kNever, ge = 0, crow = 0;
+  int nstart = kNever, ncbc = cbc, nrs = kNever, nzs = kNever, nge = 0, ncrow = 0;
+  // row symbols of even / odd stripes, loaded one stripe ahead
   int symA = sa[imin(1 + lane, leni)];
   int symB = nst > 1 ? sa[imin(65 + lane, leni)] : 0;
-  AffRow row;
-  row.ge = s_cost[(symA << 5) + kGap];
-  row.cost_row = (symA & 15) << 5;
-  // column of the lane at step st: st + cbc (current stripe), st + cbn
-  // (next stripe, from step nstart on)
-  int cbc = jlo_t - lane, cbn = 0;
-  int nstart = kNever;
-  if (nst > 1 && i + 64 <= leni) {
-    nstart = sh.S[1] + (band_lo(i + 64, newk) - band_lo(65, newk) + lane);
-    cbn = band_lo(65, newk) - sh.S[1] - lane;
+  {
+    const int i = 1 + lane;
+    nge = s_cost[(symA << 5) + kGap];
+    ncrow = (symA & 15) << 5;
+    if (i <= leni) {
+      const int lo = band_lo(i, newk);
+      nstart = lo - ncbc;
+      nrs = i + D + newk - ncbc;
+      nzs = -ncbc;
+      cells += band_hi(i, D, newk, lenj) - lo + 1;
+    }
   }
   // ---- lane 0's stripe (uniform) --------------------------------------------
-  int b0 = 0, S0 = 0, jlo0 = jlo_t;
+  int b0 = 0, S0 = 0, jlo0 = jlo_t, planned = 0;
   int next0 = nst > 1 ? __builtin_amdgcn_readfirstlane(sh.S[1]) : kNever;
   int jlon0 = band_lo(65, newk);
+  // lane 63's column offset and pending switch (uniform copies)
+  int c63 = jlo_t - 63;
+  int n63s = __builtin_amdgcn_readlane(nstart, 63), n63c = c63;
   const int blast = nst - 1;
   const int s_fin = __builtin_amdgcn_readfirstlane(sh.S[blast]) +
                     (lenj - band_lo(1 + (blast << 6), newk)) + ((leni - 1) & 63);
-  int rc = 0, rgn = 0;
+  int rc = 0, rG = 0;
   // up-left cell of lane 0's first step: row 0 at column jlo0 - 1
   int dV = kHigh, dCB = kHigh, dH = kHigh, dG = 0;
   if (jlo0 >= 1) {
     dCB = 0;
     dH = HA0;
   }
-  int lH = kHigh, lCB = kHigh, lG = 0;
   int oV = kHigh, oCB = kHigh, oH = kHigh, oG = 0;
   int cb0run = 0;
   int c0prev = sb[imax(jlo0 - 1, 0)];
@@ -333,37 +367,61 @@ __device__ void rot_round_fast(RotShared& sh, int total, const uint8_t* __restri
   // lane 0's column at step st (at most one stripe change in 8 steps)
   auto col0 = [&](int st) { return st >= next0 ? jlon0 + (st - next0) : jlo0 + (st - S0); };
   // column symbols, loaded one 4-step block ahead: cn for lane 0's column
-  // (row 0 generator), cl for the lane's own column (each lane loads its
-  // own: lanes of different stripes work on unrelated columns)
-  auto coll = [&](int st) { return imin(imax(st + (st >= nstart ? cbn : cbc), 0), lenj); };
+  // (row-0 generator of stripe 0), cl for the lane's own column (lanes of
+  // different stripes work on unrelated columns)
   int cn[4], cl[4];
 #pragma unroll
   for (int u = 0; u < 4; u++) {
-    cn[u] = sb[imin(col0(u), lenj)];
-    cl[u] = sb[coll(u)];
+    cn[u] = ld_sym(rsym, col0(u));
+    cl[u] = ld_sym(rsym, u + (u >= nstart ? ncbc : cbc));
   }
   for (int s = 0; s < total4; s += 4) {
     // lane 63's FIFO writes of the previous block precede these reads in
     // program order; the barrier keeps the compiler from moving them
     __builtin_amdgcn_wave_barrier();
-    int c0b[4], clb[4], grb[4], cdb[4];
+    // lane 0 enters stripe b0 + 1 within this block or the next: plan every
+    // lane's entry into it (all lanes are in stripe b0 by now, C1)
+    if (next0 < s + 8 && planned <= b0) {
+      planned = b0 + 1;
+      const int i = 1 + (planned << 6) + lane;
+      const int sym = (planned & 1) ? symB : symA;
+      nge = s_cost[(sym << 5) + kGap];
+      ncrow = (sym & 15) << 5;
+      nstart = kNever;
+      if (i <= leni) {
+        const int lo = band_lo(i, newk);
+        ncbc = jlon0 - next0 - lane;
+        nstart = lo - ncbc;
+        nrs = i + D + newk - ncbc;
+        nzs = -ncbc;
+        cells += band_hi(i, D, newk, lenj) - lo + 1;
+      }
+      n63s = __builtin_amdgcn_readlane(nstart, 63);
+      n63c = __builtin_amdgcn_readlane(ncbc, 63);
+      // stripe planned + 1's row symbols replace those of stripe b0, whose
+      // row constants every lane already holds
+      if (planned + 1 < nst) {
+        const int nsym = sa[imin(1 + ((planned + 1) << 6) + lane, leni)];
+        if ((planned + 1) & 1) symB = nsym;
+        else symA = nsym;
+      }
+    }
+    int c0b[4], grb[4], cdb[4];
     int4 fv[4];
     // table entries of the block's columns, looked up together: the lane's
-    // row symbol changes at its stripe switch (step nstart)
-    const int icn = ((bt + 1) & 1) ? symB : symA;
-    const int crn = (icn & 15) << 5;
+    // row changes at its switch step
 #pragma unroll
     for (int u = 0; u < 4; u++) {
       c0b[u] = cn[u];
-      clb[u] = cl[u];
       fv[u] = sh.fifo[col0(s + u) & (kRotFifo - 1)];
-      grb[u] = s_prep[clb[u]];
-      cdb[u] = s_cost[((s + u) >= nstart ? crn : row.cost_row) + (clb[u] & 15)];
+      grb[u] = s_prep[cl[u]];
+      cdb[u] = s_cost[((s + u) >= nstart ? ncrow : crow) + (cl[u] & 15)];
     }
 #pragma unroll
     for (int u = 0; u < 4; u++) {
-      cn[u] = sb[imin(imax(col0(s + 4 + u), 0), lenj)];
-      cl[u] = sb[coll(s + 4 + u)];
+      const int st = s + 4 + u;
+      if (b0 == 0) cn[u] = ld_sym(rsym, col0(st));
+      cl[u] = ld_sym(rsym, st + (st >= nstart ? ncbc : cbc));
     }
     uint32_t pack = 0;
 #pragma unroll
@@ -383,37 +441,18 @@ __device__ void rot_round_fast(RotShared& sh, int total, const uint8_t* __restri
         if (lane == 0) {
           dV = v.x; dCB = v.y; dH = v.z; dG = v.w;
         }
-        if (b0 + 1 < nst) {
-          const int nsym = sa[imin(1 + ((b0 + 1) << 6) + lane, leni)];
-          if ((b0 + 1) & 1) symB = nsym;
-          else symA = nsym;
-        }
       }
-      // every lane enters its next stripe once (divergent)
-      if (st == nstart) {
-        bt++;
-        i += 64;
-        my_lo = band_lo(i, newk);
-        my_rb = i + D + newk;
-        my_hi = band_hi(i, D, newk, lenj);
-        cells += my_hi - my_lo + 1;
-        const int ic = (bt & 1) ? symB : symA;
-        row.ge = s_cost[(ic << 5) + kGap];
-        row.cost_row = (ic & 15) << 5;
-        cbc = cbn;
-        if (bt + 1 < nst && i + 64 <= leni) {
-          const int Sn = sh.S[bt + 1], jlon = band_lo(i + 64 - lane, newk);
-          nstart = Sn + (band_lo(i + 64, newk) - jlon + lane);
-          cbn = jlon - Sn - lane;
-        } else {
-          nstart = kNever;
-        }
+      // the lane enters its next row (divergent; its first cell is this one)
+      const bool at_left = st == nstart;
+      if (at_left) {
+        cbc = ncbc; rs = nrs; zs = nzs; ge = nge; crow = ncrow;
       }
+      if (st == n63s) c63 = n63c;
       // lane 0's source row: POY's row 0 in stripe 0, else the FIFO
-      const int jl0 = jlo0 + (st - S0);
       const int c0 = c0b[u];
       int sV = kHigh, sCB, sH, sG;
       if (b0 == 0) {
+        const int jl0 = jlo0 + (st - S0);
         const int gr0 = s_prep[c0];
         const int gop0 = has_gap_opening(jl0, c0prev, c0, go);
         const int add0 = ((c0prev & kGap) && !(c0 & kGap)) ? gop0 + gr0 : gr0;
@@ -429,26 +468,45 @@ __device__ void rot_round_fast(RotShared& sh, int total, const uint8_t* __restri
       const int uCB = wave_shr1(oCB, sCB);
       const int uH = wave_shr1(oH, sH);
       const int uG = wave_shr1(oG, sG);
-      const int j = st + cbc;
-      AffCell o;
-      aff_cell_fast_core(row.ge, go, sw, j, grb[u], cdb[u], j == my_lo, j == my_rb, uV, uCB, uG,
-                         dV, dCB, dH, dG, lH, lCB, lG, o);
-      const bool inb = j >= my_lo && j <= my_hi;
-      if (inb && lane == 63) sh.fifo[j & (kRotFifo - 1)] = make_int4(o.V, o.C, o.H, o.G);
+      // the cell (aff_cell_fast_core with packed gap counts)
+      const int gr = grb[u], d = cdb[u];
+      const bool at_right = st == rs, j0 = st == zs;
+      const int hb = oCB + go;
+      const int H = at_left ? kHigh : gr + imin(oH, hb);
+      const int endH = at_left ? 0 : (oH >= hb);
+      const int vb = uCB + go;
+      const int V = at_right ? kHigh : ge + imin(uV, vb);
+      const int endV = at_right ? 0 : (uV >= vb);
+      const int m3 = imin(imin(dCB, dV), dH);
+      const int C = j0 ? kHigh : m3 + d;
+      const int a2v = !j0 && dV == m3, a2h = !j0 && dH == m3;
+      const int f = imin(imin(H, V), C);
+      const int dh = H == f, dv = V == f, da = C == f;
+      const int heqv = dh && (H == V);
+      // algn_fill_gapnum on both u16 halves at once: horizontal adds 1 to
+      // the low half, vertical to the high half
+      const int G = max_u16x2(max_u16x2(da ? dG : 0, dh ? oG + 1 : 0), dv ? uG + 0x10000 : 0);
+      const int todo = choose_dir(0, dv, dh, sw);
+      const int amode = choose_dir(0, a2v, a2h, sw);
+      const uint32_t code = (uint32_t)(todo | (amode << 2) | ((endV | heqv) << 4) |
+                                       ((endH | heqv) << 5) | ((j0 ? 0 : 1) << 6));
+      // lane 63 hands its row to lane 0 of the next stripe (columns outside
+      // its band land in slots that are rewritten before they are read)
+      if (lane == 63) sh.fifo[(st + c63) & (kRotFifo - 1)] = make_int4(V, C, H, G);
       if (st == s_fin) {
-        rc = o.f;
-        rgn = o.gmax;
+        rc = f;
+        rG = G;
       }
-      oV = o.V; oCB = o.C; oH = o.H; oG = o.G;
-      lH = o.H; lCB = o.C; lG = o.G;
+      oV = V; oCB = C; oH = H; oG = G;
       dV = uV; dCB = uCB; dH = uH; dG = uG;
-      pack |= o.code << (8 * u);
+      pack |= code << (8 * u);
     }
     reinterpret_cast<uint32_t*>(dm)[((s >> 2) << 6) + lane] = pack;
   }
   const int src = (leni - 1) & 63;
   rc_out = __shfl(rc, src);
-  rgn_out = __shfl(rgn, src);
+  const int g = __shfl(rG, src);
+  rgn_out = imax(g & 0xFFFF, (int)((unsigned)g >> 16));
 }
 
 template <bool kDm, bool kFast>
