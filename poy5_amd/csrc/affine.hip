@@ -3,6 +3,8 @@
 // stripes, Ukkonen T-doubling inside the kernel, 7-bit directions packed in
 // one byte per cell, followed by a per-pair backtrace that reproduces
 // backtrace_aff (src/algn.c:1715) and its median lookups.
+#include <type_traits>
+
 #include "poyhip_device.h"
 #include "poyhip_layout.h"
 
@@ -212,6 +214,7 @@ struct RotShared {
   int4 fifo[kRotFifo];               // V, CB, H, gap counts of row i0_b - 1
   int S[kRotMaxStripes + 1];         // stripe start steps
   int lag;                           // largest FIFO lag of the round
+  int2 fin;                          // cost and gap counts of the last cell
 };
 
 // Band of row i in the round: [lo, hi], right edge rb (src/algn.c:2113-2172).
@@ -277,14 +280,16 @@ __device__ int rot_plan(RotShared& sh, int leni, int lenj, int D, int newk, int 
   const int total = __builtin_amdgcn_readfirstlane(sh.S[nst]);
   const int lag = __builtin_amdgcn_readfirstlane(sh.lag);
   __syncthreads();
-  if (lag > kRotFifo - 72 || (long long)((total + 3) >> 2) * 4 > dm_steps_cap) return -1;
+  if (lag > kRotFifo - 72 || (long long)((total + 3) >> 2) * 4 > dm_steps_cap ||
+      total > (1 << 24))
+    return -1;
   return total;
 }
 
-// Byte loads of a pair's column sequence through a buffer resource: offsets
-// past the end (negative ones included, as unsigned) return 0 instead of
-// faulting, so a lane outside its band loads without a clamp (the cells it
-// evaluates there are never read).
+// Buffer resources over a pair's column sequence and its direction store.
+// Byte loads past the end of the sequence (negative offsets included, as
+// unsigned) return 0 instead of faulting, so a lane outside its band loads
+// without a clamp (the cells it evaluates there are never read).
 // The pointer and length are wave-uniform; the broadcasts keep the resource
 // in scalar registers.
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t seq_rsrc(const uint8_t* p, int n) {
@@ -305,6 +310,17 @@ __device__ __forceinline__ int max_u16x2(int a, int b) {
                                                             __builtin_bit_cast(rot_u16x2, b)));
 }
 
+typedef unsigned long long lmask;  // one bit per lane of the wavefront
+
+__device__ __forceinline__ lmask ballot(bool b) { return __builtin_amdgcn_ballot_w64(b); }
+
+// acc * 2 + (this lane's bit of m): one add with the lane mask as carry-in.
+__device__ __forceinline__ int shift_in(int acc, lmask m) {
+  int r;
+  asm("v_addc_co_u32_e64 %0, vcc, %1, %1, %2" : "=v"(r) : "v"(acc), "s"(m) : "vcc");
+  return r;
+}
+
 // One round of the gap-free fill on the rotating schedule.  Same outputs as
 // the plain stripe loop of k_affine_fill: cost and gap count of the last
 // cell, cells and lane-steps, stripe table and direction words.
@@ -314,15 +330,18 @@ __device__ __forceinline__ int max_u16x2(int a, int b) {
 // row as a pending record (start step nstart, same fields), planned for all
 // lanes at once when lane 0 is about to enter a stripe.  A lane switches
 // rows at step nstart, which is also the step of its row's first cell.
+template <bool kSw>
 __device__ void rot_round_fast(RotShared& sh, int total, const uint8_t* __restrict__ sa,
                                const uint8_t* __restrict__ sb, int leni, int lenj, int D, int newk,
-                               int nst, int go, int sw, int HA0, const int* s_cost,
+                               int nst, int go, int HA0, const int* s_cost,
                                const int* s_prep, uint8_t* dm, long long* stab, int& rc_out,
                                int& rgn_out, long long& cells, long long& slots) {
   const int lane = threadIdx.x;
   const int kNever = 0x7fffffff;
   for (int b = lane; b <= nst; b += 64) stab[b] = sh.S[b];
   const __amdgpu_buffer_rsrc_t rsym = seq_rsrc(sb, lenj + 1);
+  // direction words: [step / 4][lane], one dword per lane and block
+  const __amdgpu_buffer_rsrc_t rdm = seq_rsrc(dm, ((total + 3) >> 2) << 8);
   const int jlo_t = band_lo(1, newk);
   // ---- current row (none yet) and pending entry into stripe 0 -------------
   int cbc = jlo_t - lane, rs = kNever, zs = kNever, ge = 0, crow = 0;
@@ -352,7 +371,7 @@ __device__ void rot_round_fast(RotShared& sh, int total, const uint8_t* __restri
   const int blast = nst - 1;
   const int s_fin = __builtin_amdgcn_readfirstlane(sh.S[blast]) +
                     (lenj - band_lo(1 + (blast << 6), newk)) + ((leni - 1) & 63);
-  int rc = 0, rG = 0;
+  const int src = (leni - 1) & 63;   // lane of the last row
   // up-left cell of lane 0's first step: row 0 at column jlo0 - 1
   int dV = kHigh, dCB = kHigh, dH = kHigh, dG = 0;
   if (jlo0 >= 1) {
@@ -424,89 +443,122 @@ __device__ void rot_round_fast(RotShared& sh, int total, const uint8_t* __restri
       cl[u] = ld_sym(rsym, st + (st >= nstart ? ncbc : cbc));
     }
     uint32_t pack = 0;
-#pragma unroll
-    for (int u = 0; u < 4; u++) {
+    // One step.  The slow form also handles the uniform events of a block:
+    // lane 0 entering a stripe, lane 63's row switch, the round's last cell
+    // and lane 0's row-0 generator in stripe 0; quiet blocks have none.
+    auto step = [&](const int u, auto slow_t) {
+      constexpr bool kSlow = decltype(slow_t)::value;
       const int st = s + u;
-      // lane 0 enters its next stripe (uniform)
-      if (st == next0) {
-        b0++;
-        S0 = next0;
-        jlo0 = jlon0;
-        next0 = b0 + 1 < nst ? __builtin_amdgcn_readfirstlane(sh.S[b0 + 1]) : kNever;
-        jlon0 = band_lo(1 + ((b0 + 1) << 6), newk);
-        // lane 0's up-left cell: row i0 - 1 at column jlo0 - 1; the other
-        // lanes keep theirs
-        int4 v = make_int4(kHigh, kHigh, kHigh, 0);
-        if (jlo0 >= 1) v = sh.fifo[(jlo0 - 1) & (kRotFifo - 1)];
-        if (lane == 0) {
-          dV = v.x; dCB = v.y; dH = v.z; dG = v.w;
+      if constexpr (kSlow) {
+        // lane 0 enters its next stripe
+        if (st == next0) {
+          b0++;
+          S0 = next0;
+          jlo0 = jlon0;
+          next0 = b0 + 1 < nst ? __builtin_amdgcn_readfirstlane(sh.S[b0 + 1]) : kNever;
+          jlon0 = band_lo(1 + ((b0 + 1) << 6), newk);
+          // lane 0's up-left cell: row i0 - 1 at column jlo0 - 1; the other
+          // lanes keep theirs
+          int4 v = make_int4(kHigh, kHigh, kHigh, 0);
+          if (jlo0 >= 1) v = sh.fifo[(jlo0 - 1) & (kRotFifo - 1)];
+          if (lane == 0) {
+            dV = v.x; dCB = v.y; dH = v.z; dG = v.w;
+          }
         }
+        if (st == n63s) c63 = n63c;
       }
-      // the lane enters its next row (divergent; its first cell is this one)
-      const bool at_left = st == nstart;
-      if (at_left) {
-        cbc = ncbc; rs = nrs; zs = nzs; ge = nge; crow = ncrow;
-      }
-      if (st == n63s) c63 = n63c;
       // lane 0's source row: POY's row 0 in stripe 0, else the FIFO
-      const int c0 = c0b[u];
-      int sV = kHigh, sCB, sH, sG;
-      if (b0 == 0) {
-        const int jl0 = jlo0 + (st - S0);
-        const int gr0 = s_prep[c0];
-        const int gop0 = has_gap_opening(jl0, c0prev, c0, go);
-        const int add0 = ((c0prev & kGap) && !(c0 & kGap)) ? gop0 + gr0 : gr0;
-        cb0run = (jl0 == 0) ? 0 : cb0run + add0;
-        sCB = cb0run;
-        sH = (jl0 == 0) ? HA0 : cb0run;
-        sG = jl0 & 0xFFFF;
-      } else {
-        sV = fv[u].x; sCB = fv[u].y; sH = fv[u].z; sG = fv[u].w;
+      int sV = fv[u].x, sCB = fv[u].y, sH = fv[u].z, sG = fv[u].w;
+      if constexpr (kSlow) {
+        const int c0 = c0b[u];
+        if (b0 == 0) {
+          const int jl0 = jlo0 + (st - S0);
+          const int gr0 = s_prep[c0];
+          const int gop0 = has_gap_opening(jl0, c0prev, c0, go);
+          const int add0 = ((c0prev & kGap) && !(c0 & kGap)) ? gop0 + gr0 : gr0;
+          cb0run = (jl0 == 0) ? 0 : cb0run + add0;
+          sV = kHigh;
+          sCB = cb0run;
+          sH = (jl0 == 0) ? HA0 : cb0run;
+          sG = jl0 & 0xFFFF;
+        }
+        c0prev = c0;
       }
-      c0prev = c0;
       const int uV = wave_shr1(oV, sV);
       const int uCB = wave_shr1(oCB, sCB);
       const int uH = wave_shr1(oH, sH);
       const int uG = wave_shr1(oG, sG);
-      // the cell (aff_cell_fast_core with packed gap counts)
+      // the lane enters its next row (divergent; its first cell is this
+      // one).  Its left inputs become HIGH_NUM and HIGH_NUM + 1 after the
+      // lane below has read its outputs: the first cell's horizontal state
+      // is then HIGH_NUM plus the gap cost, above every finite cost, and its
+      // END_HORIZONTAL flag is clear, as at the band's left edge.
+      if (st == nstart) {
+        cbc = ncbc; rs = nrs; zs = nzs; ge = nge; crow = ncrow;
+        oH = kHigh;
+        oCB = kHigh + 1;
+      }
+      // the cell (aff_cell_fast_core with packed gap counts); tie and end
+      // flags are lane masks, combined on the scalar unit.  Flags the
+      // backtrace never reads are left unmasked: column 0 (the walk stops
+      // there), END_VERTICAL at the right edge and END_HORIZONTAL at the
+      // left edge (a run reaching them would extend a HIGH_NUM state), and
+      // END_BLOCK (no gap-gap block move in a gap-free pair).
       const int gr = grb[u], d = cdb[u];
       const bool at_right = st == rs, j0 = st == zs;
       const int hb = oCB + go;
-      const int H = at_left ? kHigh : gr + imin(oH, hb);
-      const int endH = at_left ? 0 : (oH >= hb);
+      const int H = gr + imin(oH, hb);
       const int vb = uCB + go;
       const int V = at_right ? kHigh : ge + imin(uV, vb);
-      const int endV = at_right ? 0 : (uV >= vb);
       const int m3 = imin(imin(dCB, dV), dH);
       const int C = j0 ? kHigh : m3 + d;
-      const int a2v = !j0 && dV == m3, a2h = !j0 && dH == m3;
       const int f = imin(imin(H, V), C);
-      const int dh = H == f, dv = V == f, da = C == f;
-      const int heqv = dh && (H == V);
+      const bool dh = H == f, dv = V == f, da = C == f;
+      const lmask mdh = ballot(dh), mdv = ballot(dv);
+      const lmask ma2v = ballot(dV == m3), ma2h = ballot(dH == m3);
+      const lmask mheqv = mdh & ballot(H == V);
+      const lmask mendV = ballot(uV >= vb) | mheqv;
+      const lmask mendH = ballot(oH >= hb) | mheqv;
       // algn_fill_gapnum on both u16 halves at once: horizontal adds 1 to
       // the low half, vertical to the high half
       const int G = max_u16x2(max_u16x2(da ? dG : 0, dh ? oG + 1 : 0), dv ? uG + 0x10000 : 0);
-      const int todo = choose_dir(0, dv, dh, sw);
-      const int amode = choose_dir(0, a2v, a2h, sw);
-      const uint32_t code = (uint32_t)(todo | (amode << 2) | ((endV | heqv) << 4) |
-                                       ((endH | heqv) << 5) | ((j0 ? 0 : 1) << 6));
+      // direction byte, high bit first: ends, then choose_dir of the align
+      // mode and of the todo mode (bit 0: not vertical, or both vertical and
+      // horizontal with the swap flag; bit 1: neither)
+      int code = shift_in(0, mendH);
+      code = shift_in(code, mendV);
+      code = shift_in(code, ~(ma2v | ma2h));
+      code = shift_in(code, kSw ? ~ma2v | ma2h : ~ma2v);
+      code = shift_in(code, ~(mdv | mdh));
+      code = shift_in(code, kSw ? ~mdv | mdh : ~mdv);
       // lane 63 hands its row to lane 0 of the next stripe (columns outside
       // its band land in slots that are rewritten before they are read)
       if (lane == 63) sh.fifo[(st + c63) & (kRotFifo - 1)] = make_int4(V, C, H, G);
-      if (st == s_fin) {
-        rc = f;
-        rG = G;
+      // the last cell of the round: cost and gap counts
+      if constexpr (kSlow) {
+        if (st == s_fin) {
+          if (lane == src) sh.fin = make_int2(f, G);
+        }
       }
       oV = V; oCB = C; oH = H; oG = G;
       dV = uV; dCB = uCB; dH = uH; dG = uG;
-      pack |= code << (8 * u);
+      pack = u == 0 ? (uint32_t)code : pack | ((uint32_t)code << (8 * u));
+    };
+    const bool quiet = next0 >= s + 4 && (unsigned)(n63s - s) >= 4u &&
+                       (unsigned)(s_fin - s) >= 4u && b0 != 0;
+    if (quiet) {
+      step(0, std::false_type{}); step(1, std::false_type{});
+      step(2, std::false_type{}); step(3, std::false_type{});
+    } else {
+      step(0, std::true_type{}); step(1, std::true_type{});
+      step(2, std::true_type{}); step(3, std::true_type{});
     }
-    reinterpret_cast<uint32_t*>(dm)[((s >> 2) << 6) + lane] = pack;
+    __builtin_amdgcn_raw_buffer_store_b32(pack, rdm, lane << 2, (s >> 2) << 8, 0);
   }
-  const int src = (leni - 1) & 63;
-  rc_out = __shfl(rc, src);
-  const int g = __shfl(rG, src);
-  rgn_out = imax(g & 0xFFFF, (int)((unsigned)g >> 16));
+  __builtin_amdgcn_wave_barrier();
+  const int2 fin = sh.fin;
+  rc_out = fin.x;
+  rgn_out = imax(fin.y & 0xFFFF, (int)((unsigned)fin.y >> 16));
 }
 
 template <bool kDm, bool kFast>
@@ -585,9 +637,14 @@ __global__ __launch_bounds__(64) void k_affine_fill(AffFillArgs A) {
       int rot_total = -1;
       if constexpr (kFast && kDm) {
         rot_total = rot_plan(s_rot, leni, lenj, D, newk, nstripes, aff_dm_bytes(La, Lb) >> 6);
-        if (rot_total > 0)
-          rot_round_fast(s_rot, rot_total, sa, sb, leni, lenj, D, newk, nstripes, go, sw, HA0,
-                         s_cost, s_prep, dm, stab, rc, rgn, cells, slots);
+        if (rot_total > 0) {
+          if (sw)
+            rot_round_fast<true>(s_rot, rot_total, sa, sb, leni, lenj, D, newk, nstripes, go, HA0,
+                                 s_cost, s_prep, dm, stab, rc, rgn, cells, slots);
+          else
+            rot_round_fast<false>(s_rot, rot_total, sa, sb, leni, lenj, D, newk, nstripes, go, HA0,
+                                  s_cost, s_prep, dm, stab, rc, rgn, cells, slots);
+        }
       }
       for (int b = 0; rot_total <= 0 && b < nstripes; b++) {
         const int i0 = 1 + (b << 6);
