@@ -334,8 +334,8 @@ template <bool kSw>
 __device__ void rot_round_fast(RotShared& sh, int total, const uint8_t* __restrict__ sa,
                                const uint8_t* __restrict__ sb, int leni, int lenj, int D, int newk,
                                int nst, int go, int HA0, const int* s_cost,
-                               const int* s_prep, uint8_t* dm, long long* stab, int& rc_out,
-                               int& rgn_out, long long& cells, long long& slots) {
+                               const int* s_ge, const int* s_prep, uint8_t* dm, long long* stab,
+                               int& rc_out, int& rgn_out, long long& cells, long long& slots) {
   const int lane = threadIdx.x;
   const int kNever = 0x7fffffff;
   for (int b = lane; b <= nst; b += 64) stab[b] = sh.S[b];
@@ -351,8 +351,8 @@ __device__ void rot_round_fast(RotShared& sh, int total, const uint8_t* __restri
   int symB = nst > 1 ? sa[imin(65 + lane, leni)] : 0;
   {
     const int i = 1 + lane;
-    nge = s_cost[(symA << 5) + kGap];
-    ncrow = (symA & 15) << 5;
+    nge = s_ge[symA];
+    ncrow = (symA & 15) << 4;
     if (i <= leni) {
       const int lo = band_lo(i, newk);
       nstart = lo - ncbc;
@@ -404,8 +404,8 @@ __device__ void rot_round_fast(RotShared& sh, int total, const uint8_t* __restri
       planned = b0 + 1;
       const int i = 1 + (planned << 6) + lane;
       const int sym = (planned & 1) ? symB : symA;
-      nge = s_cost[(sym << 5) + kGap];
-      ncrow = (sym & 15) << 5;
+      nge = s_ge[sym];
+      ncrow = (sym & 15) << 4;
       nstart = kNever;
       if (i <= leni) {
         const int lo = band_lo(i, newk);
@@ -561,15 +561,23 @@ __device__ void rot_round_fast(RotShared& sh, int total, const uint8_t* __restri
   rgn_out = imax(fin.y & 0xFFFF, (int)((unsigned)fin.y >> 16));
 }
 
+// At most 128 VGPRs: four wavefronts per SIMD (shared memory allows as many)
 template <bool kDm, bool kFast>
-__global__ __launch_bounds__(64) void k_affine_fill(AffFillArgs A) {
+__global__ __launch_bounds__(64, 4) void k_affine_fill(AffFillArgs A) {
   // gap-free pairs only index rows 0..15 of the cost table
-  constexpr int kCostN = kFast ? 512 : 1024;
+  // gap-free pairs only index rows and columns 0..15 of the cost table,
+  // kept 16 wide (the wavefront's shared memory stays within a sixteenth
+  // of the CU's, for four wavefronts per SIMD)
+  constexpr int kCostN = kFast ? 256 : 1024;
+  constexpr int kRowShift = kFast ? 4 : 5;
   __shared__ int s_cost[kCostN];
+  __shared__ int s_ge[32];
   __shared__ int s_prep[32];
   __shared__ RotShared s_rot;
   const int lane = threadIdx.x;
-  for (int k = lane; k < kCostN; k += 64) s_cost[k] = A.cm->cost[k];
+  for (int k = lane; k < kCostN; k += 64)
+    s_cost[k] = A.cm->cost[((k >> kRowShift) << 5) + (k & ((1 << kRowShift) - 1))];
+  if (lane < 32) s_ge[lane] = A.cm->cost[(lane << 5) + kGap];
   if (lane < 32) s_prep[lane] = A.cm->prepend[lane];
   __syncthreads();
   const int go = A.cm->gap_open;
@@ -640,10 +648,10 @@ __global__ __launch_bounds__(64) void k_affine_fill(AffFillArgs A) {
         if (rot_total > 0) {
           if (sw)
             rot_round_fast<true>(s_rot, rot_total, sa, sb, leni, lenj, D, newk, nstripes, go, HA0,
-                                 s_cost, s_prep, dm, stab, rc, rgn, cells, slots);
+                                 s_cost, s_ge, s_prep, dm, stab, rc, rgn, cells, slots);
           else
             rot_round_fast<false>(s_rot, rot_total, sa, sb, leni, lenj, D, newk, nstripes, go, HA0,
-                                  s_cost, s_prep, dm, stab, rc, rgn, cells, slots);
+                                  s_cost, s_ge, s_prep, dm, stab, rc, rgn, cells, slots);
         }
       }
       for (int b = 0; rot_total <= 0 && b < nstripes; b++) {
@@ -666,10 +674,10 @@ __global__ __launch_bounds__(64) void k_affine_fill(AffFillArgs A) {
         row.gap = ic & kGap;
         row.isng = (ic & kGap) == 0;
         row.ip_nogap = (ip & kGap) == 0;
-        row.ge = s_cost[(ic << 5) + kGap];
+        row.ge = s_ge[ic];
         row.go = has_gap_opening(ir, ip, ic, go);
         row.vext = (ir > 1 && (ip & kGap) && !(ic & kGap)) ? row.go + row.ge : row.ge;
-        row.cost_row = (ic & 15) << 5;
+        row.cost_row = (ic & 15) << kRowShift;
         const int my_lo = act ? imax(i - newk, 0) : (1 << 30);
         const int my_rb = i + D + newk;
         const int my_hi = act ? ((my_rb <= lenj - 1) ? my_rb : lenj) : -1;
