@@ -586,6 +586,18 @@ namespace {
 bool op_is_affine(int op) { return op == POYHIP_OP_ALIGN_AFFINE || op == POYHIP_OP_COST_AFFINE; }
 bool op_has_dm(int op) { return op == POYHIP_OP_ALIGN_AFFINE || op == POYHIP_OP_ALIGN_2D; }
 
+// Direction-store budget of a batch when the caller gives none: the free
+// device memory less an eighth of the device (at least 8 GiB) left to other
+// allocations.  On a 288 GB MI355X a batch of 65,536 pairs of up to 5,000
+// symbols then runs as one chunk, so the persistent fill balances its
+// largest pairs against the whole batch instead of half of it.
+size_t auto_dm_budget() {
+  size_t fr = 0, tot = 0;
+  if (hipMemGetInfo(&fr, &tot) != hipSuccess) return (size_t)4 << 30;
+  const size_t reserve = std::max(tot / 8, (size_t)8 << 30);
+  return fr > 2 * reserve ? fr - reserve : fr / 2;
+}
+
 int64_t pair_dm_bytes(int op, int la, int lb) {
   if (op == POYHIP_OP_ALIGN_AFFINE) return aff_dm_bytes(la, lb);
   if (op == POYHIP_OP_ALIGN_2D) return lin_dm_bytes(la, lb);
@@ -810,11 +822,7 @@ int poyhip_batch_create(poyhip_ctx* c, int op, int n, const int* len1, const int
     so += n_stripes(b->la[k]) + 1;
   }
   // chunks by direction-store budget
-  if (dm_budget == 0) {
-    size_t fr = 0, tot = 0;
-    if (hipMemGetInfo(&fr, &tot) != hipSuccess) fr = (size_t)8 << 30;
-    dm_budget = std::min(fr / 2, (size_t)96 << 30);
-  }
+  if (dm_budget == 0) dm_budget = auto_dm_budget();
   b->dm_budget = (int64_t)dm_budget;
   if (op == POYHIP_OP_ALIGN_2D) {
     int64_t est = 0;
