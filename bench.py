@@ -349,6 +349,11 @@ def main():
     }
     if roofline is None:
         result["roofline_note"] = "graph replay: per-kernel events unavailable, run with --graph 0"
+    # the headline batch's direction store is sized to most of the device:
+    # release it before the other sections plan their own
+    b.close()
+    del cost, out, outlen
+    torch.cuda.empty_cache()
     if not args.no_plain:
         print("[bench] plain per-call sample", file=sys.stderr, flush=True)
         al = Align(ctx)
@@ -375,7 +380,6 @@ def main():
         torch.cuda.synchronize()
         result["copy_calibration_bytes"] = nb
         del x, y
-    b.close()
     ctx.close()
     print(json.dumps(result), flush=True)
 
