@@ -121,9 +121,9 @@ __device__ __forceinline__ void aff_cell(const AffRow& r, int go, int sw, int j,
 // aff_cell for a pair whose symbols past the leading gap carry no gap bit:
 // then HAS_GAP_OPENING is gap_open for every row and column past 0, the
 // horizontal/vertical extensions are the plain gap costs, the gap-gap block
-// state stays >= HIGH_NUM so it never reaches a minimum or a tie, and its
-// END_BLOCK flag is set on every column past 0.  What remains is Gotoh's
-// recurrence with POY's tie flags.
+// state stays >= HIGH_NUM so it never reaches a minimum or a tie (the walk
+// never enters it, so its END_BLOCK flag is not stored).  What remains is
+// Gotoh's recurrence with POY's tie flags.
 // The column's table entries come in: gr = prepend[jc] (horizontal gap
 // cost of the column symbol) and d = cost(row symbol, jc).
 __device__ __forceinline__ void aff_cell_fast_core(int ge, int go, int sw, int j, int gr, int d,
@@ -139,7 +139,6 @@ __device__ __forceinline__ void aff_cell_fast_core(int ge, int go, int sw, int j
   const int m3 = imin(imin(dCB, dV), dH);
   const bool j0 = (j == 0);
   const int C = j0 ? kHigh : m3 + d;
-  const int a2v = !j0 && dV == m3, a2h = !j0 && dH == m3;
   const int f = imin(imin(H, V), C);
   const int dh = H == f, dv = V == f, da = C == f;
   const int heqv = dh && (H == V);
@@ -150,10 +149,10 @@ __device__ __forceinline__ void aff_cell_fast_core(int ge, int go, int sw, int j
   const int g2 = imax(imax(da ? pg2l : 0, dh ? g2l : 0), dv ? pg2 + 1 : 0);
   o.G = (g1 & 0xFFFF) | ((g2 & 0xFFFF) << 16);
   o.gmax = imax(g1 & 0xFFFF, g2 & 0xFFFF);
+  // gap-free direction nibble: todo move, vertical end, horizontal end (the
+  // align-mode move equals the diagonal predecessor's todo move)
   const int todo = choose_dir(0, dv, dh, sw);
-  const int amode = choose_dir(0, a2v, a2h, sw);
-  o.code = (uint32_t)(todo | (amode << 2) | ((endV | heqv) << 4) | ((endH | heqv) << 5) |
-                      ((j0 ? 0 : 1) << 6));
+  o.code = (uint32_t)(todo | ((endV | heqv) << 2) | ((endH | heqv) << 3));
   o.V = V;
   o.C = C;
   o.H = H;
@@ -233,10 +232,12 @@ __device__ __forceinline__ int band_hi(int i, int D, int newk, int lenj) {
 //       block before the one holding S_{b+1} (one block ahead, for the column
 //       symbol loads), and lane 0 reads FIFO entries up front per block, at
 //       least 4 steps after lane 63 wrote them.
-//   C2: lane t enters stripe b+1 after its own last cell of stripe b.
-//       Lane t+1 reads lane t's outputs through DPP one step later, so it
-//       still sees lane t's last cell of stripe b; at its own last cell
-//       (the band's right edge) it ignores the row above.
+//   C2: lane t's first cell of stripe b+1 comes at least 4 steps after its
+//       last cell of stripe b, so it can take its new row at the 4-step
+//       block boundary before that first cell.  Lane t+1 reads lane t's
+//       outputs through DPP one step later, so it still sees lane t's last
+//       cell of stripe b; at its own last cell (the band's right edge) it
+//       ignores the row above.
 // The FIFO is safe while lag = S_{b+1} - S_b - (jlo_{b+1} - jlo_b) - 63
 // stays within kRotFifo - 72 (no entry is overwritten before it is read).
 // Gap counts travel as two u16 halves of one word with 32-bit adds, exact
@@ -254,7 +255,7 @@ __device__ int rot_plan(RotShared& sh, int leni, int lenj, int D, int newk, int 
       int c2 = 0;
       for (int t = 0; t < 64 && i0 + t + 64 <= leni; t++)
         c2 = imax(c2, band_hi(i0 + t, D, newk, lenj) - band_lo(i0 + t + 64, newk));
-      c2 += (jlon - jlo) + 1;
+      c2 += (jlon - jlo) + 4;
       d = imax(71 + (jlon - jlo), c2);
       lag = d - (jlon - jlo) - 63;
     } else {
@@ -323,55 +324,61 @@ __device__ __forceinline__ int shift_in(int acc, lmask m) {
 
 // One round of the gap-free fill on the rotating schedule.  Same outputs as
 // the plain stripe loop of k_affine_fill: cost and gap count of the last
-// cell, cells and lane-steps, stripe table and direction words.
+// cell, cells and lane-steps, stripe table and direction nibbles.
 //
 // Every lane keeps its current row in step coordinates (column st + cbc,
-// right edge at step rs, column 0 at step zs) and the entry into its next
-// row as a pending record (start step nstart, same fields), planned for all
-// lanes at once when lane 0 is about to enter a stripe.  A lane switches
-// rows at step nstart, which is also the step of its row's first cell.
+// right edge at step rs, column 0 at step zs) and the next row as a pending
+// record, planned for all lanes at once when lane 0 is about to enter a
+// stripe.  A lane takes its pending row at the 4-step block boundary at or
+// before the row's first cell (step fst): its previous row ended at least 4
+// steps before fst (C2), so whole blocks see one row per lane and the
+// block's table lookups and column loads need no per-step selection.  Cells
+// a lane evaluates before fst lie left of its band and are never read.
+//
+// Direction nibbles: bits 0-1 the todo move (choose_dir), bit 2
+// END_VERTICAL | HORIZONTAL_EQ_VERTICAL, bit 3 END_HORIZONTAL |
+// HORIZONTAL_EQ_VERTICAL; the align-mode move of a cell is the todo move of
+// its diagonal predecessor, where the backtrace reads it.  Layout: per 4-step
+// block one 16-bit word per lane, step u of the block in bits 15-4u..12-4u.
 template <bool kSw>
 __device__ void rot_round_fast(RotShared& sh, int total, const uint8_t* __restrict__ sa,
                                const uint8_t* __restrict__ sb, int leni, int lenj, int D, int newk,
-                               int nst, int go, int HA0, const int* s_cost,
-                               const int* s_ge, const int* s_prep, uint8_t* dm, long long* stab,
-                               int& rc_out, int& rgn_out, long long& cells, long long& slots) {
+                               int nst, int go, int HA0, const int* s_cost, const int* s_ge,
+                               const int* s_prep, uint8_t* dm, long long* stab, int& rc_out,
+                               int& rgn_out, long long& cells, long long& slots) {
   const int lane = threadIdx.x;
   const int kNever = 0x7fffffff;
   for (int b = lane; b <= nst; b += 64) stab[b] = sh.S[b];
   const __amdgpu_buffer_rsrc_t rsym = seq_rsrc(sb, lenj + 1);
-  // direction words: [step / 4][lane], one dword per lane and block
-  const __amdgpu_buffer_rsrc_t rdm = seq_rsrc(dm, ((total + 3) >> 2) << 8);
+  const __amdgpu_buffer_rsrc_t rdm = seq_rsrc(dm, ((total + 3) >> 2) << 7);
   const int jlo_t = band_lo(1, newk);
-  // ---- current row (none yet) and pending entry into stripe 0 -------------
-  int cbc = jlo_t - lane, rs = kNever, zs = kNever, ge = 0, crow = 0;
-  int nstart = kNever, ncbc = cbc, nrs = kNever, nzs = kNever, nge = 0, ncrow = 0;
-  // row symbols of even / odd stripes, loaded one stripe ahead
-  int symA = sa[imin(1 + lane, leni)];
-  int symB = nst > 1 ? sa[imin(65 + lane, leni)] : 0;
+  // ---- current row: stripe 0 -----------------------------------------------
+  int symA = sa[imin(1 + lane, leni)];  // row symbols of even / odd stripes,
+  int symB = nst > 1 ? sa[imin(65 + lane, leni)] : 0;  // one stripe ahead
+  int cbc = jlo_t - lane, rs = kNever, zs = kNever, fst = kNever;
+  int ge = s_ge[symA], crow = (symA & 15) << 4;
   {
     const int i = 1 + lane;
-    nge = s_ge[symA];
-    ncrow = (symA & 15) << 4;
     if (i <= leni) {
       const int lo = band_lo(i, newk);
-      nstart = lo - ncbc;
-      nrs = i + D + newk - ncbc;
-      nzs = -ncbc;
+      fst = lo - cbc;
+      rs = i + D + newk - cbc;
+      zs = -cbc;
       cells += band_hi(i, D, newk, lenj) - lo + 1;
     }
   }
+  // ---- pending row (none yet) ----------------------------------------------
+  int nsw = kNever, ncbc = 0, nrs = kNever, nzs = kNever, nge = 0, ncrow = 0;
   // ---- lane 0's stripe (uniform) --------------------------------------------
   int b0 = 0, S0 = 0, jlo0 = jlo_t, planned = 0;
   int next0 = nst > 1 ? __builtin_amdgcn_readfirstlane(sh.S[1]) : kNever;
   int jlon0 = band_lo(65, newk);
   // lane 63's column offset and pending switch (uniform copies)
-  int c63 = jlo_t - 63;
-  int n63s = __builtin_amdgcn_readlane(nstart, 63), n63c = c63;
+  int c63 = jlo_t - 63, n63sw = kNever, n63c = 0;
   const int blast = nst - 1;
   const int s_fin = __builtin_amdgcn_readfirstlane(sh.S[blast]) +
                     (lenj - band_lo(1 + (blast << 6), newk)) + ((leni - 1) & 63);
-  const int src = (leni - 1) & 63;   // lane of the last row
+  const int src = (leni - 1) & 63;  // lane of the last row
   // up-left cell of lane 0's first step: row 0 at column jlo0 - 1
   int dV = kHigh, dCB = kHigh, dH = kHigh, dG = 0;
   if (jlo0 >= 1) {
@@ -392,7 +399,7 @@ __device__ void rot_round_fast(RotShared& sh, int total, const uint8_t* __restri
 #pragma unroll
   for (int u = 0; u < 4; u++) {
     cn[u] = ld_sym(rsym, col0(u));
-    cl[u] = ld_sym(rsym, u + (u >= nstart ? ncbc : cbc));
+    cl[u] = ld_sym(rsym, u + cbc);
   }
   for (int s = 0; s < total4; s += 4) {
     // lane 63's FIFO writes of the previous block precede these reads in
@@ -406,16 +413,17 @@ __device__ void rot_round_fast(RotShared& sh, int total, const uint8_t* __restri
       const int sym = (planned & 1) ? symB : symA;
       nge = s_ge[sym];
       ncrow = (sym & 15) << 4;
-      nstart = kNever;
+      ncbc = jlon0 - next0 - lane;
+      nsw = kNever;
       if (i <= leni) {
         const int lo = band_lo(i, newk);
-        ncbc = jlon0 - next0 - lane;
-        nstart = lo - ncbc;
+        fst = lo - ncbc;
+        nsw = fst & ~3;
         nrs = i + D + newk - ncbc;
         nzs = -ncbc;
         cells += band_hi(i, D, newk, lenj) - lo + 1;
       }
-      n63s = __builtin_amdgcn_readlane(nstart, 63);
+      n63sw = __builtin_amdgcn_readlane(nsw, 63);
       n63c = __builtin_amdgcn_readlane(ncbc, 63);
       // stripe planned + 1's row symbols replace those of stripe b0, whose
       // row constants every lane already holds
@@ -425,27 +433,34 @@ __device__ void rot_round_fast(RotShared& sh, int total, const uint8_t* __restri
         else symA = nsym;
       }
     }
+    // the lane takes its pending row at this block boundary
+    if (s == nsw) {
+      cbc = ncbc; rs = nrs; zs = nzs; ge = nge; crow = ncrow;
+    }
+    if (s == n63sw) c63 = n63c;
     int c0b[4], grb[4], cdb[4];
     int4 fv[4];
-    // table entries of the block's columns, looked up together: the lane's
-    // row changes at its switch step
+    // table entries of the block's columns, looked up together
 #pragma unroll
     for (int u = 0; u < 4; u++) {
       c0b[u] = cn[u];
       fv[u] = sh.fifo[col0(s + u) & (kRotFifo - 1)];
       grb[u] = s_prep[cl[u]];
-      cdb[u] = s_cost[((s + u) >= nstart ? ncrow : crow) + (cl[u] & 15)];
+      cdb[u] = s_cost[crow + (cl[u] & 15)];
     }
+    {
+      const int cbn = (s + 4 >= nsw) ? ncbc : cbc;
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
-      const int st = s + 4 + u;
-      if (b0 == 0) cn[u] = ld_sym(rsym, col0(st));
-      cl[u] = ld_sym(rsym, st + (st >= nstart ? ncbc : cbc));
+      for (int u = 0; u < 4; u++) {
+        const int st = s + 4 + u;
+        if (b0 == 0) cn[u] = ld_sym(rsym, col0(st));
+        cl[u] = ld_sym(rsym, st + cbn);
+      }
     }
-    uint32_t pack = 0;
+    int pack = 0;
     // One step.  The slow form also handles the uniform events of a block:
-    // lane 0 entering a stripe, lane 63's row switch, the round's last cell
-    // and lane 0's row-0 generator in stripe 0; quiet blocks have none.
+    // lane 0 entering a stripe, the round's last cell and lane 0's row-0
+    // generator in stripe 0; quiet blocks have none.
     auto step = [&](const int u, auto slow_t) {
       constexpr bool kSlow = decltype(slow_t)::value;
       const int st = s + u;
@@ -465,7 +480,6 @@ __device__ void rot_round_fast(RotShared& sh, int total, const uint8_t* __restri
             dV = v.x; dCB = v.y; dH = v.z; dG = v.w;
           }
         }
-        if (st == n63s) c63 = n63c;
       }
       // lane 0's source row: POY's row 0 in stripe 0, else the FIFO
       int sV = fv[u].x, sCB = fv[u].y, sH = fv[u].z, sG = fv[u].w;
@@ -488,26 +502,15 @@ __device__ void rot_round_fast(RotShared& sh, int total, const uint8_t* __restri
       const int uCB = wave_shr1(oCB, sCB);
       const int uH = wave_shr1(oH, sH);
       const int uG = wave_shr1(oG, sG);
-      // the lane enters its next row (divergent; its first cell is this
-      // one).  Its left inputs become HIGH_NUM and HIGH_NUM + 1 after the
-      // lane below has read its outputs: the first cell's horizontal state
-      // is then HIGH_NUM plus the gap cost, above every finite cost, and its
-      // END_HORIZONTAL flag is clear, as at the band's left edge.
-      if (st == nstart) {
-        cbc = ncbc; rs = nrs; zs = nzs; ge = nge; crow = ncrow;
-        oH = kHigh;
-        oCB = kHigh + 1;
-      }
       // the cell (aff_cell_fast_core with packed gap counts); tie and end
       // flags are lane masks, combined on the scalar unit.  Flags the
       // backtrace never reads are left unmasked: column 0 (the walk stops
       // there), END_VERTICAL at the right edge and END_HORIZONTAL at the
-      // left edge (a run reaching them would extend a HIGH_NUM state), and
-      // END_BLOCK (no gap-gap block move in a gap-free pair).
+      // left edge (a run reaching them would extend a HIGH_NUM state).
       const int gr = grb[u], d = cdb[u];
-      const bool at_right = st == rs, j0 = st == zs;
+      const bool at_left = st == fst, at_right = st == rs, j0 = st == zs;
       const int hb = oCB + go;
-      const int H = gr + imin(oH, hb);
+      const int H = at_left ? kHigh : gr + imin(oH, hb);
       const int vb = uCB + go;
       const int V = at_right ? kHigh : ge + imin(uV, vb);
       const int m3 = imin(imin(dCB, dV), dH);
@@ -515,22 +518,17 @@ __device__ void rot_round_fast(RotShared& sh, int total, const uint8_t* __restri
       const int f = imin(imin(H, V), C);
       const bool dh = H == f, dv = V == f, da = C == f;
       const lmask mdh = ballot(dh), mdv = ballot(dv);
-      const lmask ma2v = ballot(dV == m3), ma2h = ballot(dH == m3);
       const lmask mheqv = mdh & ballot(H == V);
-      const lmask mendV = ballot(uV >= vb) | mheqv;
-      const lmask mendH = ballot(oH >= hb) | mheqv;
       // algn_fill_gapnum on both u16 halves at once: horizontal adds 1 to
       // the low half, vertical to the high half
       const int G = max_u16x2(max_u16x2(da ? dG : 0, dh ? oG + 1 : 0), dv ? uG + 0x10000 : 0);
-      // direction byte, high bit first: ends, then choose_dir of the align
-      // mode and of the todo mode (bit 0: not vertical, or both vertical and
-      // horizontal with the swap flag; bit 1: neither)
-      int code = shift_in(0, mendH);
-      code = shift_in(code, mendV);
-      code = shift_in(code, ~(ma2v | ma2h));
-      code = shift_in(code, kSw ? ~ma2v | ma2h : ~ma2v);
-      code = shift_in(code, ~(mdv | mdh));
-      code = shift_in(code, kSw ? ~mdv | mdh : ~mdv);
+      // direction nibble, high bit first: horizontal end, vertical end,
+      // then choose_dir of the todo mode (bit 1: neither vertical nor
+      // horizontal; bit 0: not vertical, or both with the swap flag)
+      pack = shift_in(pack, ballot(oH >= hb) | mheqv);
+      pack = shift_in(pack, ballot(uV >= vb) | mheqv);
+      pack = shift_in(pack, ~(mdv | mdh));
+      pack = shift_in(pack, kSw ? ~mdv | mdh : ~mdv);
       // lane 63 hands its row to lane 0 of the next stripe (columns outside
       // its band land in slots that are rewritten before they are read)
       if (lane == 63) sh.fifo[(st + c63) & (kRotFifo - 1)] = make_int4(V, C, H, G);
@@ -542,10 +540,8 @@ __device__ void rot_round_fast(RotShared& sh, int total, const uint8_t* __restri
       }
       oV = V; oCB = C; oH = H; oG = G;
       dV = uV; dCB = uCB; dH = uH; dG = uG;
-      pack = u == 0 ? (uint32_t)code : pack | ((uint32_t)code << (8 * u));
     };
-    const bool quiet = next0 >= s + 4 && (unsigned)(n63s - s) >= 4u &&
-                       (unsigned)(s_fin - s) >= 4u && b0 != 0;
+    const bool quiet = next0 >= s + 4 && (unsigned)(s_fin - s) >= 4u && b0 != 0;
     if (quiet) {
       step(0, std::false_type{}); step(1, std::false_type{});
       step(2, std::false_type{}); step(3, std::false_type{});
@@ -553,7 +549,7 @@ __device__ void rot_round_fast(RotShared& sh, int total, const uint8_t* __restri
       step(0, std::true_type{}); step(1, std::true_type{});
       step(2, std::true_type{}); step(3, std::true_type{});
     }
-    __builtin_amdgcn_raw_buffer_store_b32(pack, rdm, lane << 2, (s >> 2) << 8, 0);
+    __builtin_amdgcn_raw_buffer_store_b16((unsigned short)pack, rdm, lane << 1, (s >> 2) << 7, 0);
   }
   __builtin_amdgcn_wave_barrier();
   const int2 fin = sh.fin;
@@ -644,7 +640,7 @@ __global__ __launch_bounds__(64, 4) void k_affine_fill(AffFillArgs A) {
       int rc = 0, rgn = 0;
       int rot_total = -1;
       if constexpr (kFast && kDm) {
-        rot_total = rot_plan(s_rot, leni, lenj, D, newk, nstripes, aff_dm_bytes(La, Lb) >> 6);
+        rot_total = rot_plan(s_rot, leni, lenj, D, newk, nstripes, aff_dm_bytes(La, Lb) >> 5);
         if (rot_total > 0) {
           if (sw)
             rot_round_fast<true>(s_rot, rot_total, sa, sb, leni, lenj, D, newk, nstripes, go, HA0,
@@ -663,7 +659,8 @@ __global__ __launch_bounds__(64, 4) void k_affine_fill(AffFillArgs A) {
         const int jhi = (ilast + D + newk <= lenj - 1) ? ilast + D + newk : lenj;
         const int nsteps = (jhi - jlo + 1) + (ilast - i0);
         const int nsteps4 = (nsteps + 3) & ~3;
-        if (kDm && lane == 0) stab[b] = dmpos >> 6;  // stripe start in steps
+        // stripe start in steps (gap-free pairs: 32 direction bytes a step)
+        if (kDm && lane == 0) stab[b] = dmpos >> (kFast ? 5 : 6);
         slots += (long long)nsteps * 64;
 
         // row constants of lane t (row i); rows past leni get the last row's
@@ -756,11 +753,19 @@ __global__ __launch_bounds__(64, 4) void k_affine_fill(AffFillArgs A) {
             lH = o.H; lCB = o.C; lG = o.G;
             dV = uV; dCB = uCB; dH = uH; dED = uED; dG = uG;
             jprev = jc;
-            pack |= o.code << (8 * u);
+            if constexpr (kFast)
+              pack = (pack << 4) | o.code;  // nibbles, step 0 highest
+            else
+              pack |= o.code << (8 * u);
           }
-          if (kDm) reinterpret_cast<uint32_t*>(dm + dmpos)[((s >> 2) << 6) + lane] = pack;
+          if constexpr (kDm) {
+            if constexpr (kFast)
+              reinterpret_cast<uint16_t*>(dm + dmpos)[((s >> 2) << 6) + lane] = (uint16_t)pack;
+            else
+              reinterpret_cast<uint32_t*>(dm + dmpos)[((s >> 2) << 6) + lane] = pack;
+          }
         }
-        dmpos += (long long)(nsteps4 >> 2) * 256;
+        dmpos += (long long)(nsteps4 >> 2) * (kFast ? 128 : 256);
         __threadfence_block();
       }
       const int src = (leni - 1) & 63;
@@ -780,7 +785,7 @@ __global__ __launch_bounds__(64, 4) void k_affine_fill(AffFillArgs A) {
       r.gap_num = gap_num;
       r.newk = newk;
       r.rounds = rounds;
-      r.full = 0;
+      r.full = kFast ? 1 : 0;  // direction format: 1 = gap-free nibbles
       r.pad = 0;
       r.cells = cells;
       r.slots = slots;
@@ -829,6 +834,11 @@ __global__ __launch_bounds__(64) void k_affine_backtrace(AffBtArgs A) {
   int cb = -1, jlo = 0;
   long long base = 0;
 #define PUT(buf, n, v) (buf)[cap - 1 - (n)++] = (uint8_t)(v)
+  // direction format: bytes (see the top of this file) or, for gap-free
+  // pairs, nibbles (todo move, vertical end, horizontal end) whose align-mode
+  // move is the todo move of the diagonal predecessor
+  const bool nib = A.res[pid].full != 0;
+  const int vend = nib ? 4 : 16, hend = nib ? 8 : 32;
   while (i != 0 && j != 0) {
     const int bb = (i - 1) >> 6;
     if (bb != cb) {
@@ -838,17 +848,26 @@ __global__ __launch_bounds__(64) void k_affine_backtrace(AffBtArgs A) {
     }
     const int t = (i - 1) & 63;
     const int s = j - jlo + t;
-    // stab[b] = global step of the stripe's first step; words [step/4][lane]
+    // stab[b] = global step of the stripe's first step; per 4-step block one
+    // word per lane (bytes: 4 bytes, nibbles: 2 bytes)
     const long long g = base + s;
-    long long at = ((g >> 2) << 8) + (t << 2) + (g & 3);
-    // cells on the path are inside the band; the clamp only keeps a corrupt
-    // walk inside the pair's region
-    if (s < 0 || at >= dm_cap) at = 0;
-    const int code = dm[at];
-    if (mode == 4) {
-      mode = code & 3;
-    } else if (mode == 0) {
-      if (code & 16) mode = 4;
+    int code;
+    if (nib) {
+      long long at = ((g >> 2) << 7) + (t << 1);
+      // cells on the path are inside the band; the clamp only keeps a
+      // corrupt walk inside the pair's region
+      if (s < 0 || at + 1 >= dm_cap) at = 0;
+      const int w = *reinterpret_cast<const uint16_t*>(dm + at);
+      code = (w >> (12 - 4 * (int)(g & 3))) & 15;
+    } else {
+      long long at = ((g >> 2) << 8) + (t << 2) + (g & 3);
+      if (s < 0 || at >= dm_cap) at = 0;
+      code = dm[at];
+    }
+    // a todo read decides the move of this same cell
+    if (mode == 4) mode = code & 3;
+    if (mode == 0) {
+      if (code & vend) mode = 4;
       if (!(ic & kGap)) { PUT(md, nm, ic | kGap); PUT(mw, nw, ic | kGap); }
       else PUT(mw, nw, kGap);
       PUT(ri, ni, ic);
@@ -856,7 +875,7 @@ __global__ __launch_bounds__(64) void k_affine_backtrace(AffBtArgs A) {
       i--;
       ic = sa[i];
     } else if (mode == 1) {
-      if (code & 32) mode = 4;
+      if (code & hend) mode = 4;
       if (!(jc & kGap)) { PUT(md, nm, jc | kGap); PUT(mw, nw, jc | kGap); }
       else PUT(mw, nw, kGap);
       PUT(ri, ni, kGap);
@@ -872,7 +891,7 @@ __global__ __launch_bounds__(64) void k_affine_backtrace(AffBtArgs A) {
       jc = sb[j];
       ic = sa[i];
     } else {
-      mode = (code >> 2) & 3;
+      mode = nib ? 4 : (code >> 2) & 3;
       const int prep = med[((ic & 15) << 5) + (jc & 15)];
       PUT(md, nm, prep);
       PUT(mw, nw, prep);

@@ -75,7 +75,8 @@ struct PairResult {
   int gap_num;
   int newk;       // band half-width of the final round (-1: no DP rows)
   int rounds;
-  int full;       // linear: 1 when the full plane was filled
+  int full;       // linear: 1 when the full plane was filled; affine: 1 when
+                  // the directions are gap-free nibbles (else bytes)
   int pad;
   long long cells;  // cells evaluated over all rounds
   long long slots;  // lane-steps issued by the fill (cells + idle lanes)
