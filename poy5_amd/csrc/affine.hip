@@ -210,7 +210,9 @@ constexpr int kRotFifo = 512;        // entries (power of two)
 constexpr int kRotMaxStripes = 128;
 
 struct RotShared {
-  int4 fifo[kRotFifo];               // V, CB, H, gap counts of row i0_b - 1
+  // V, CB, H and gap counts of row i0_b - 1, one array each (lane 0 reads
+  // each component into its own register)
+  int fV[kRotFifo], fC[kRotFifo], fH[kRotFifo], fG[kRotFifo];
   int S[kRotMaxStripes + 1];         // stripe start steps
   int lag;                           // largest FIFO lag of the round
   int2 fin;                          // cost and gap counts of the last cell
@@ -264,7 +266,7 @@ __device__ int rot_plan(RotShared& sh, int leni, int lenj, int D, int newk, int 
         d = imax(d, band_hi(i0 + t, D, newk, lenj) - jlo + t + 1);
     }
     sh.S[b + 1] = d;
-    sh.fifo[b].x = lag;  // scratch: the FIFO is idle between rounds
+    sh.fV[b] = lag;  // scratch: the FIFO is idle between rounds
   }
   __syncthreads();
   if (lane == 0) {
@@ -273,7 +275,7 @@ __device__ int rot_plan(RotShared& sh, int leni, int lenj, int D, int newk, int 
     for (int b = 0; b < nst; b++) {
       acc += sh.S[b + 1];
       sh.S[b + 1] = acc;
-      mlag = imax(mlag, sh.fifo[b].x);
+      mlag = imax(mlag, sh.fV[b]);
     }
     sh.lag = mlag;
   }
@@ -439,12 +441,11 @@ __device__ void rot_round_fast(RotShared& sh, int total, const uint8_t* __restri
     }
     if (s == n63sw) c63 = n63c;
     int c0b[4], grb[4], cdb[4];
-    int4 fv[4];
+    int fvx[4], fvy[4], fvz[4], fvw[4];
     // table entries of the block's columns, looked up together
 #pragma unroll
     for (int u = 0; u < 4; u++) {
       c0b[u] = cn[u];
-      fv[u] = sh.fifo[col0(s + u) & (kRotFifo - 1)];
       grb[u] = s_prep[cl[u]];
       cdb[u] = s_cost[crow + (cl[u] & 15)];
     }
@@ -460,7 +461,7 @@ __device__ void rot_round_fast(RotShared& sh, int total, const uint8_t* __restri
     int pack = 0;
     // One step.  The slow form also handles the uniform events of a block:
     // lane 0 entering a stripe, the round's last cell and lane 0's row-0
-    // generator in stripe 0; quiet blocks have none.
+    // generator in stripe 0, and column-0 cells; quiet blocks have none.
     auto step = [&](const int u, auto slow_t) {
       constexpr bool kSlow = decltype(slow_t)::value;
       const int st = s + u;
@@ -475,14 +476,17 @@ __device__ void rot_round_fast(RotShared& sh, int total, const uint8_t* __restri
           // lane 0's up-left cell: row i0 - 1 at column jlo0 - 1; the other
           // lanes keep theirs
           int4 v = make_int4(kHigh, kHigh, kHigh, 0);
-          if (jlo0 >= 1) v = sh.fifo[(jlo0 - 1) & (kRotFifo - 1)];
+          if (jlo0 >= 1) {
+            const int x = (jlo0 - 1) & (kRotFifo - 1);
+            v = make_int4(sh.fV[x], sh.fC[x], sh.fH[x], sh.fG[x]);
+          }
           if (lane == 0) {
             dV = v.x; dCB = v.y; dH = v.z; dG = v.w;
           }
         }
       }
       // lane 0's source row: POY's row 0 in stripe 0, else the FIFO
-      int sV = fv[u].x, sCB = fv[u].y, sH = fv[u].z, sG = fv[u].w;
+      int sV = fvx[u], sCB = fvy[u], sH = fvz[u], sG = fvw[u];
       if constexpr (kSlow) {
         const int c0 = c0b[u];
         if (b0 == 0) {
@@ -508,7 +512,8 @@ __device__ void rot_round_fast(RotShared& sh, int total, const uint8_t* __restri
       // there), END_VERTICAL at the right edge and END_HORIZONTAL at the
       // left edge (a run reaching them would extend a HIGH_NUM state).
       const int gr = grb[u], d = cdb[u];
-      const bool at_left = st == fst, at_right = st == rs, j0 = st == zs;
+      const bool at_left = st == fst, at_right = st == rs;
+      const bool j0 = kSlow && st == zs;  // quiet blocks hold no column-0 cell
       const int hb = oCB + go;
       const int H = at_left ? kHigh : gr + imin(oH, hb);
       const int vb = uCB + go;
@@ -531,7 +536,13 @@ __device__ void rot_round_fast(RotShared& sh, int total, const uint8_t* __restri
       pack = shift_in(pack, kSw ? ~mdv | mdh : ~mdv);
       // lane 63 hands its row to lane 0 of the next stripe (columns outside
       // its band land in slots that are rewritten before they are read)
-      if (lane == 63) sh.fifo[(st + c63) & (kRotFifo - 1)] = make_int4(V, C, H, G);
+      if (lane == 63) {
+        const int x = (st + c63) & (kRotFifo - 1);
+        sh.fV[x] = V;
+        sh.fC[x] = C;
+        sh.fH[x] = H;
+        sh.fG[x] = G;
+      }
       // the last cell of the round: cost and gap counts
       if constexpr (kSlow) {
         if (st == s_fin) {
@@ -541,11 +552,28 @@ __device__ void rot_round_fast(RotShared& sh, int total, const uint8_t* __restri
       oV = V; oCB = C; oH = H; oG = G;
       dV = uV; dCB = uCB; dH = uH; dG = uG;
     };
-    const bool quiet = next0 >= s + 4 && (unsigned)(s_fin - s) >= 4u && b0 != 0;
+    // lane 0's FIFO entries of the block, read at the start of each form:
+    // one register per component, each the DPP destination of its up value
+    // (lane 0 keeps it)
+    auto read_fifo = [&]() {
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int x = col0(s + u) & (kRotFifo - 1);
+        fvx[u] = sh.fV[x];
+        fvy[u] = sh.fC[x];
+        fvz[u] = sh.fH[x];
+        fvw[u] = sh.fG[x];
+      }
+    };
+    // quiet: no uniform event and no lane at column 0 in this block
+    const bool quiet = next0 >= s + 4 && (unsigned)(s_fin - s) >= 4u && b0 != 0 &&
+                       ballot((unsigned)(zs - s) < 4u) == 0;
     if (quiet) {
+      read_fifo();
       step(0, std::false_type{}); step(1, std::false_type{});
       step(2, std::false_type{}); step(3, std::false_type{});
     } else {
+      read_fifo();
       step(0, std::true_type{}); step(1, std::true_type{});
       step(2, std::true_type{}); step(3, std::true_type{});
     }
