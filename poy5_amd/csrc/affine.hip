@@ -29,6 +29,7 @@ struct AffFillArgs {
   PairResult* res;
   int* counter;
   const uint8_t* cls;  // per-pair class (1: gap-free fast path), or null
+  const int* ngen;     // number of general-class pairs, or null
 };
 
 // Reads `p[j]` for j in [0, n); other j read p[0] (the value is unused).
@@ -172,15 +173,24 @@ __device__ __forceinline__ void aff_cell_fast(const AffRow& r, int go, int sw, i
 
 // 1 when a pair can take aff_cell_fast: no symbol past the leading gap of
 // either sequence carries the gap bit.
-__global__ void k_affine_classify(PairTable pt, uint8_t* cls) {
-  const int pid = blockIdx.x * blockDim.x + threadIdx.x;
+// One wavefront per pair, consecutive bytes across the lanes.  ngen (may be
+// null) counts the pairs of the general class.
+__global__ __launch_bounds__(64) void k_affine_classify(PairTable pt, uint8_t* cls, int* ngen) {
+  const int pid = blockIdx.x;
   if (pid >= pt.npairs) return;
   const uint8_t* sa = pt.blob + rows_off(pt, pid);
   const uint8_t* sb = pt.blob + cols_off(pt, pid);
+  const int la = pt.len_a[pid], lb = pt.len_b[pid];
   int any = 0;
-  for (int k = 1; k < pt.len_a[pid]; k++) any |= sa[k];
-  for (int k = 1; k < pt.len_b[pid]; k++) any |= sb[k];
-  cls[pid] = (any & kGap) ? 0 : 1;
+#pragma unroll 4
+  for (int k = 1 + (int)threadIdx.x; k < la; k += 64) any |= sa[k];
+#pragma unroll 4
+  for (int k = 1 + (int)threadIdx.x; k < lb; k += 64) any |= sb[k];
+  const bool gap = __builtin_amdgcn_ballot_w64((any & kGap) != 0) != 0;
+  if (threadIdx.x == 0) {
+    cls[pid] = gap ? 0 : 1;
+    if (gap && ngen != nullptr) atomicAdd(ngen, 1);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -599,6 +609,8 @@ __global__ __launch_bounds__(64, 4) void k_affine_fill(AffFillArgs A) {
   __shared__ int s_prep[32];
   __shared__ RotShared s_rot;
   const int lane = threadIdx.x;
+  // the general kernel has nothing to do when no pair carries a gap bit
+  if (!kFast && A.ngen != nullptr && __builtin_amdgcn_readfirstlane(*A.ngen) == 0) return;
   for (int k = lane; k < kCostN; k += 64)
     s_cost[k] = A.cm->cost[((k >> kRowShift) << 5) + (k & ((1 << kRowShift) - 1))];
   if (lane < 32) s_ge[lane] = A.cm->cost[(lane << 5) + kGap];
@@ -836,6 +848,11 @@ struct AffBtArgs {
 // the packed direction bytes.  Outputs are written right to left into four
 // regions of capacity La+Lb+2 (resi, resj, median, medianwg).
 __global__ __launch_bounds__(64) void k_affine_backtrace(AffBtArgs A) {
+  // median table rows 0..15 in shared memory: align columns look it up
+  // without a global round trip on the walk's critical path
+  __shared__ uint8_t s_med[512];
+  for (int k = threadIdx.x; k < 512; k += 64) s_med[k] = A.cm->median[k];
+  __syncthreads();
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= A.pt.n) return;
   const int pid = A.pt.order[q];
@@ -855,7 +872,7 @@ __global__ __launch_bounds__(64) void k_affine_backtrace(AffBtArgs A) {
   const long long* stab = A.stab + A.pt.stab_off[pid];
   const int newk = A.res[pid].newk;
   const long long dm_cap = aff_dm_bytes(La, Lb);
-  const uint8_t* med = A.cm->median;
+  const uint8_t* med = s_med;
   int i = La - 1, j = Lb - 1;
   int ic = sa[i], jc = sb[j];
   int mode = 4;  // 0 vert, 1 horz, 2 diag, 3 align, 4 todo

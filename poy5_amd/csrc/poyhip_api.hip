@@ -434,6 +434,7 @@ int poyhip_align_affine_3(poyhip_ctx* c, const uint8_t* si, int li, const uint8_
   A.res = dres;
   A.counter = counter;
   A.cls = nullptr;
+  A.ngen = nullptr;
   int any = 0;
   for (int k = 1; k < li; k++) any |= si[k];
   for (int k = 1; k < lj; k++) any |= sj[k];
@@ -674,11 +675,14 @@ int batch_enqueue(poyhip_batch* b, const uint8_t* blob, const int64_t* off1, con
   hipStream_t st = c->stream;
   const int nch = (int)b->chunks.size();
   if (timing) HIPCHK(c, hipEventRecord(b->ev[2 * nch], st));
-  hipLaunchKernelGGL(k_zero_ints, dim3((2 * nch + 63) / 64), dim3(64), 0, st, b->d_counters,
-                     2 * nch);
+  // counters: two per chunk (fast and general fill queues), then the number
+  // of general-class pairs
+  hipLaunchKernelGGL(k_zero_ints, dim3((2 * nch + 1 + 63) / 64), dim3(64), 0, st, b->d_counters,
+                     2 * nch + 1);
   if (b->op == POYHIP_OP_ALIGN_AFFINE) {
     PairTable all = batch_table(b, blob, off1, off2, -1);
-    hipLaunchKernelGGL(k_affine_classify, dim3((b->n + 255) / 256), dim3(256), 0, st, all, b->d_cls);
+    hipLaunchKernelGGL(k_affine_classify, dim3(b->n), dim3(64), 0, st, all, b->d_cls,
+                       b->d_counters + 2 * nch);
   }
   if (!op_is_affine(b->op)) {
     PairTable all = batch_table(b, blob, off1, off2, -1);
@@ -711,6 +715,7 @@ int batch_enqueue(poyhip_batch* b, const uint8_t* blob, const int64_t* off1, con
           A.stab = b->d_stab;
           A.res = b->d_res;
           A.cls = b->d_cls;
+          A.ngen = b->d_counters + 2 * nch;
           A.counter = b->d_counters + 2 * k;
           hipLaunchKernelGGL((k_affine_fill<true, true>), dim3(b->grid), dim3(64), 0, st, A);
           A.counter = b->d_counters + 2 * k + 1;

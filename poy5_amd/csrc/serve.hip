@@ -280,7 +280,7 @@ int srv_launch(poyhip_server* s, SrvSlot& sl) {
   const long long wss = (ws_ints(op, maxlb) + 63) & ~63LL;
   hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, st, sl.d_counters, 2);
   if (op == POYHIP_OP_ALIGN_AFFINE)
-    hipLaunchKernelGGL(k_affine_classify, dim3((n + 255) / 256), dim3(256), 0, st, pt, sl.d_cls);
+    hipLaunchKernelGGL(k_affine_classify, dim3(n), dim3(64), 0, st, pt, sl.d_cls, nullptr);
   if (!op_is_affine(op))
     hipLaunchKernelGGL(k_linear_deltawh, dim3((n + 255) / 256), dim3(256), 0, st, pt, dt.dw, 2,
                        dt.dwin);
@@ -294,6 +294,7 @@ int srv_launch(poyhip_server* s, SrvSlot& sl) {
     A.stab = sl.d_stab;
     A.res = sl.d_res;
     A.cls = sl.d_cls;
+    A.ngen = nullptr;
     A.counter = sl.d_counters;
     hipLaunchKernelGGL((k_affine_fill<true, true>), dim3(grid), dim3(64), 0, st, A);
     A.counter = sl.d_counters + 1;
