@@ -675,6 +675,21 @@ __global__ __launch_bounds__(64, 4) void k_affine_fill(AffFillArgs A) {
     for (;;) {
       const int p = (T - D) / 2;
       newk = p >= leni ? leni - 1 : p;
+      // A round whose stop test cannot pass leaves only its row-0 state
+      // change: the last cell's gap count is at least D (every path makes D
+      // more horizontal than vertical moves, and the count keeps the most
+      // horizontal moves over tied paths, exact below 2^16 columns), so
+      // gap_num < p needs p > D.  Gap-free pairs only: the general kernel's
+      // rounds also leave its row-0 gap-gap buffer behind.
+      if constexpr (kFast) {
+        const int newp = (2 * T - D) / 2;
+        if (D >= p && newp - lenj + 1 < 0 && lenj <= 65535) {
+          rounds++;
+          if (imin(leni, newk) >= 2) HA0 = kHigh;
+          T *= 2;
+          continue;
+        }
+      }
       const int nstripes = (leni + 63) >> 6;
       long long dmpos = 0;
       int rc = 0, rgn = 0;
