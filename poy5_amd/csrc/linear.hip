@@ -84,6 +84,14 @@ __global__ __launch_bounds__(64) void k_linear_fill(LinFillArgs A) {
     for (;;) {
       Tfin = T;
       const int p = (T - Dd) / 2;
+      // A band round whose stop test cannot pass is skipped: its gap count
+      // is the deletion count plus Dd, so (gap_num + 1) < p needs p > Dd + 1,
+      // and rounds leave no state behind.
+      if (!full && A.phase != 2 && p <= Dd + 1 && (2 * T - Dd) / 2 - L2 + 1 < 0) {
+        rounds++;
+        T *= 2;
+        continue;
+      }
       newk = full ? L1 - 1 : (p >= L1 ? L1 - 1 : p);
       const int lfr = full ? L2 : imin(L2 - L1 + 1 + p, L2);
       if (lane == 0) cells += lfr;
