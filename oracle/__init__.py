@@ -41,6 +41,7 @@ class OrcStats(C.Structure):
     _fields_ = [
         ("cells", C.c_int64), ("rounds", C.c_int), ("full_plane", C.c_int),
         ("final_T", C.c_int), ("final_k", C.c_int), ("gap_num", C.c_int),
+        ("min_gap_excess", C.c_int),
     ]
 
 

@@ -355,6 +355,7 @@ int orc_nw(const uint8_t *s1, int L1, const uint8_t *s2, int L2,
     orc_stats local;
     if (!st) st = &local;
     memset(st, 0, sizeof *st);
+    st->min_gap_excess = INT_MAX;
     int *prec = precalc_2d(cm, s2, L2);
     int *mm = (int *)malloc(sizeof(int) * (size_t)L1 * L2);
     /* algn_nw_limit (2954): width 50, height 50 + (L1 - L2) */
@@ -384,6 +385,7 @@ int orc_nw(const uint8_t *s1, int L1, const uint8_t *s2, int L2,
             st->final_T = T;
             st->final_k = p >= L1 ? L1 - 1 : p;
             st->gap_num = gap_num;
+            if (gap_num - (L2 - L1) < st->min_gap_excess) st->min_gap_excess = gap_num - (L2 - L1);
             int newp = (2 * T - (L2 - L1)) / 2;
             if ((gap_num + 1) < p || newp - L2 + 1 >= 0) break;
             T *= 2;
@@ -907,6 +909,7 @@ int orc_align_affine_3(const uint8_t *si, int Li, const uint8_t *sj, int Lj,
     orc_stats local;
     if (!st) st = &local;
     memset(st, 0, sizeof *st);
+    st->min_gap_excess = INT_MAX;
     if (Lj < Li) return -1; /* "pass the shorter one as first" */
     int largest = Lj + 1;
     size_t msz = (size_t)(Li + 12) * (Lj + 1);
@@ -961,6 +964,8 @@ int orc_align_affine_3(const uint8_t *si, int Li, const uint8_t *sj, int Lj,
         st->final_T = T;
         st->final_k = p >= x.leni ? x.leni - 1 : p;
         st->gap_num = gap_num;
+        if (gap_num - (x.lenj - x.leni) < st->min_gap_excess)
+            st->min_gap_excess = gap_num - (x.lenj - x.leni);
         int newp = (2 * T - (x.lenj - x.leni)) / 2;
         if (gap_num < p || newp - x.lenj + 1 >= 0) break;
         T *= 2;

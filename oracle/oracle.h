@@ -85,6 +85,8 @@ typedef struct {
     int final_T;       /* last Ukkonen threshold */
     int final_k;       /* last newk */
     int gap_num;       /* last gap count used by the stop test */
+    int min_gap_excess; /* min over band rounds of gap count - (L2 - L1);
+                           INT_MAX when no band round ran */
 } orc_stats;
 
 /* ---- tables (src/cost_matrix.ml, src/cm.c) ---- */

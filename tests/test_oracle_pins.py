@@ -164,6 +164,31 @@ def test_affine_band_rounds_agree_with_full():
     assert multi > 5
 
 
+@pytest.mark.parametrize("model", [(1, 1, 3), (1, 2, 1), (2, 3, 4), (1, 1, 0)])
+@pytest.mark.parametrize("gapbit", [0.0, 0.2])
+def test_gap_count_at_least_length_difference(model, gapbit):
+    """Every band round's gap count is at least D = len2 - len1: each path
+    makes D more horizontal than vertical moves, and the count keeps the
+    most moves of a kind over tied paths.  The fill kernels skip rounds
+    whose stop test therefore cannot pass (affine gap_num < p with p <= D,
+    linear gap_num + 1 < p with p <= D + 1); this pins the premise on the
+    oracle's rounds, affine and linear, with and without gap-bit symbols."""
+    rng = np.random.default_rng(21 + model[2] + int(10 * gapbit))
+    acm, lcm = oracle.cm_dna(*model), oracle.cm_dna(model[0], model[1])
+    rounds = 0
+    for t in range(30):
+        a = rseq(rng, int(rng.integers(1, 300)), 0.05, gapbit)
+        b = mutate(rng, a, int(rng.integers(0, 60)))
+        if len(b) < len(a):
+            a, b = b, a
+        st = oracle.align_affine_3(a, b, acm, t & 1)["stats"]
+        assert st.min_gap_excess >= 0, (t, st.min_gap_excess)
+        rounds += st.rounds
+        st2 = oracle.align_2(a, b, lcm)[3]
+        assert st2.full_plane or st2.min_gap_excess >= 0, (t, st2.min_gap_excess)
+    assert rounds > 60
+
+
 def test_affine_medians_follow_alignment():
     """median_with_gaps is the column-wise median table of the alignment;
     the median drops the gap columns (inputs without gap-bit symbols)."""
