@@ -859,18 +859,57 @@ struct AffBtArgs {
   int* outlen;  // 4 per pair: aligned length, aligned length, median, median-with-gaps
 };
 
-// backtrace_aff (src/algn.c:1715-1819) with follow_*_affine (1531-1592) on
-// the packed direction bytes.  Outputs are written right to left into four
-// regions of capacity La+Lb+2 (resi, resj, median, medianwg).
+// This lane's direction word in global 4-step block q of a pair's store
+// (bytes: 32-bit words, nibbles: 16-bit words); 0 outside the store.
+__device__ __forceinline__ uint32_t dm_word(const uint8_t* dm, long long cap, bool nib,
+                                            long long q) {
+  const int wlog = nib ? 1 : 2;
+  const long long at = (q << (6 + wlog)) + ((long long)threadIdx.x << wlog);
+  if (q < 0 || at + (1 << wlog) > cap) return 0u;
+  return nib ? (uint32_t)*reinterpret_cast<const uint16_t*>(dm + at)
+             : *reinterpret_cast<const uint32_t*>(dm + at);
+}
+
+// Exclusive prefix sum over the 64 lanes (Hillis-Steele with shuffles) and
+// the total.
+__device__ __forceinline__ int wave_excl_sum(int x, int& total) {
+  int v = x;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int y = __shfl_up(v, d);
+    if ((int)threadIdx.x >= d) v += y;
+  }
+  total = __builtin_amdgcn_readlane(v, 63);
+  return v - x;
+}
+
+// backtrace_aff (src/algn.c:1715-1819) with follow_*_affine (1531-1592),
+// one wavefront per pair, in two passes.
+//
+// Walk: uniform; the mode machine needs only the direction codes, so the
+// symbols stay out of it.  The wavefront keeps the direction words of 4
+// consecutive 4-step blocks in registers (block q: lane t holds the word of
+// row t), the next 4 loading behind them.  The walk visits cells at strictly
+// decreasing global steps (every move lowers the step within a stripe, and
+// the previous stripe's cells of a column come earlier), so the windows only
+// move down and the walk waits for memory about once per window.  Each step
+// records its move (0 vertical, 1 horizontal, 2 gap-gap diagonal, 3 align)
+// at output position cap-1-step of the first region, 64 steps per store.
+//
+// Expansion: 64 steps per pass over the lanes.  The row and column of step s
+// are the start minus the moves before it (a prefix sum); the four outputs
+// of a step follow from its move and the two symbols exactly as the walk's
+// puts in the reference do (resi, resj and medianwg take one byte per step,
+// the median only where it keeps a column, compacted by a second prefix
+// sum).  Outputs are written right to left into four regions of capacity
+// La+Lb+2 (resi, resj, median, medianwg).
 __global__ __launch_bounds__(64) void k_affine_backtrace(AffBtArgs A) {
-  // median table rows 0..15 in shared memory: align columns look it up
-  // without a global round trip on the walk's critical path
+  // median table rows 0..15 in shared memory
   __shared__ uint8_t s_med[512];
-  for (int k = threadIdx.x; k < 512; k += 64) s_med[k] = A.cm->median[k];
+  const int lane = threadIdx.x;
+  for (int k = lane; k < 512; k += 64) s_med[k] = A.cm->median[k];
   __syncthreads();
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= A.pt.n) return;
-  const int pid = A.pt.order[q];
+  const int pid = __builtin_amdgcn_readfirstlane(A.pt.order[blockIdx.x]);
   const uint8_t* sa = A.pt.blob + rows_off(A.pt, pid);
   const uint8_t* sb = A.pt.blob + cols_off(A.pt, pid);
   const int La = A.pt.len_a[pid], Lb = A.pt.len_b[pid];
@@ -882,23 +921,31 @@ __global__ __launch_bounds__(64) void k_affine_backtrace(AffBtArgs A) {
   uint8_t* rj = geo ? blk : blk + cap;
   uint8_t* md = blk + 2 * cap;
   uint8_t* mw = blk + 3 * cap;
-  int ni = 0, nj = 0, nm = 0, nw = 0;
   const uint8_t* dm = A.dm + A.pt.dm_off[pid];
   const long long* stab = A.stab + A.pt.stab_off[pid];
-  const int newk = A.res[pid].newk;
+  const PairResult R = A.res[pid];
+  const int newk = R.newk;
   const long long dm_cap = aff_dm_bytes(La, Lb);
-  const uint8_t* med = s_med;
-  int i = La - 1, j = Lb - 1;
-  int ic = sa[i], jc = sb[j];
-  int mode = 4;  // 0 vert, 1 horz, 2 diag, 3 align, 4 todo
-  int cb = -1, jlo = 0;
-  long long base = 0;
-#define PUT(buf, n, v) (buf)[cap - 1 - (n)++] = (uint8_t)(v)
   // direction format: bytes (see the top of this file) or, for gap-free
   // pairs, nibbles (todo move, vertical end, horizontal end) whose align-mode
   // move is the todo move of the diagonal predecessor
-  const bool nib = A.res[pid].full != 0;
+  const bool nib = R.full != 0;
   const int vend = nib ? 4 : 16, hend = nib ? 8 : 32;
+
+  // ---- walk ----------------------------------------------------------------
+  uint32_t wa0 = 0, wa1 = 0, wa2 = 0, wa3 = 0, wb0 = 0, wb1 = 0, wb2 = 0, wb3 = 0;
+  long long qa = -(1ll << 60), qb = -(1ll << 60);  // top blocks of the windows
+  uint32_t mv = 0;  // move of step 64m + lane of the current chunk m
+  int n = 0;        // steps recorded
+  auto record = [&](int m) __attribute__((always_inline)) {
+    if (lane == (n & 63)) mv = (uint32_t)m;
+    n++;
+    if ((n & 63) == 0) ri[cap - 1 - (n - 64) - lane] = (uint8_t)mv;
+  };
+  int i = La - 1, j = Lb - 1;
+  int mode = 4;  // 0 vert, 1 horz, 2 diag, 3 align, 4 todo
+  int cb = -1, jlo = 0;
+  long long base = 0;
   while (i != 0 && j != 0) {
     const int bb = (i - 1) >> 6;
     if (bb != cb) {
@@ -908,85 +955,120 @@ __global__ __launch_bounds__(64) void k_affine_backtrace(AffBtArgs A) {
     }
     const int t = (i - 1) & 63;
     const int s = j - jlo + t;
-    // stab[b] = global step of the stripe's first step; per 4-step block one
-    // word per lane (bytes: 4 bytes, nibbles: 2 bytes)
-    const long long g = base + s;
-    int code;
-    if (nib) {
-      long long at = ((g >> 2) << 7) + (t << 1);
-      // cells on the path are inside the band; the clamp only keeps a
-      // corrupt walk inside the pair's region
-      if (s < 0 || at + 1 >= dm_cap) at = 0;
-      const int w = *reinterpret_cast<const uint16_t*>(dm + at);
-      code = (w >> (12 - 4 * (int)(g & 3))) & 15;
-    } else {
-      long long at = ((g >> 2) << 8) + (t << 2) + (g & 3);
-      if (s < 0 || at >= dm_cap) at = 0;
-      code = dm[at];
+    // stab[b] = global step of the stripe's first step; cells on the path are
+    // inside the band (a step before the stripe reads code 0)
+    int code = 0;
+    if (s >= 0) {
+      const long long g = base + s;
+      const long long q = g >> 2;
+      if (q > qa || q < qa - 3) {
+        if (q <= qb && q >= qb - 3) {
+          wa0 = wb0; wa1 = wb1; wa2 = wb2; wa3 = wb3;
+          qa = qb;
+        } else {
+          wa0 = dm_word(dm, dm_cap, nib, q);
+          wa1 = dm_word(dm, dm_cap, nib, q - 1);
+          wa2 = dm_word(dm, dm_cap, nib, q - 2);
+          wa3 = dm_word(dm, dm_cap, nib, q - 3);
+          qa = q;
+        }
+        // the next window loads while this one is walked
+        wb0 = dm_word(dm, dm_cap, nib, qa - 4);
+        wb1 = dm_word(dm, dm_cap, nib, qa - 5);
+        wb2 = dm_word(dm, dm_cap, nib, qa - 6);
+        wb3 = dm_word(dm, dm_cap, nib, qa - 7);
+        qb = qa - 4;
+      }
+      const int k = (int)(qa - q);
+      const uint32_t v = k < 2 ? (k == 0 ? wa0 : wa1) : (k == 2 ? wa2 : wa3);
+      const uint32_t w = __builtin_amdgcn_readlane(v, t);
+      code = nib ? (int)(w >> (12 - 4 * (int)(g & 3))) & 15 : (int)(w >> (8 * (int)(g & 3))) & 255;
     }
     // a todo read decides the move of this same cell
     if (mode == 4) mode = code & 3;
+    record(mode);
     if (mode == 0) {
       if (code & vend) mode = 4;
-      if (!(ic & kGap)) { PUT(md, nm, ic | kGap); PUT(mw, nw, ic | kGap); }
-      else PUT(mw, nw, kGap);
-      PUT(ri, ni, ic);
-      PUT(rj, nj, kGap);
       i--;
-      ic = sa[i];
     } else if (mode == 1) {
       if (code & hend) mode = 4;
-      if (!(jc & kGap)) { PUT(md, nm, jc | kGap); PUT(mw, nw, jc | kGap); }
-      else PUT(mw, nw, kGap);
-      PUT(ri, ni, kGap);
-      PUT(rj, nj, jc);
       j--;
-      jc = sb[j];
     } else if (mode == 2) {
       if (code & 64) mode = 4;
-      PUT(ri, ni, ic);
-      PUT(rj, nj, jc);
-      PUT(mw, nw, kGap);
       i--; j--;
-      jc = sb[j];
-      ic = sa[i];
     } else {
       mode = nib ? 4 : (code >> 2) & 3;
-      const int prep = med[((ic & 15) << 5) + (jc & 15)];
-      PUT(md, nm, prep);
-      PUT(mw, nw, prep);
-      PUT(ri, ni, ic);
-      PUT(rj, nj, jc);
       i--; j--;
-      jc = sb[j];
-      ic = sa[i];
     }
   }
-  while (i != 0) {
-    if (!(ic & kGap)) { PUT(md, nm, ic | kGap); PUT(mw, nw, ic | kGap); }
-    else PUT(mw, nw, kGap);
-    PUT(ri, ni, ic);
-    PUT(rj, nj, kGap);
-    i--;
-    ic = sa[i];
+  // the reference's trailing loops: vertical to row 0, then horizontal
+  while (i != 0) { record(0); i--; }
+  while (j != 0) { record(1); j--; }
+  if ((n & 63) != 0 && lane < (n & 63)) ri[cap - 1 - (n & ~63) - lane] = (uint8_t)mv;
+  __threadfence_block();
+
+  // ---- expansion -----------------------------------------------------------
+  int ci = 0, cj = 0, cm = 0;  // rows, columns and median bytes before the chunk
+  int mlast = -1;              // last median byte written
+  for (int c = 0; c < n; c += 64) {
+    const int s = c + lane;
+    const bool ok = s < n;
+    const int m = ok ? ri[cap - 1 - s] : 1;
+    // moves before this one, packed: rows in bits 0-7, columns in 8-15
+    const int di = ok && m != 1, dj = ok && m != 0;
+    int tot;
+    const int ex = wave_excl_sum(di | (dj << 8), tot);
+    const int ii = La - 1 - ci - (ex & 255), jj = Lb - 1 - cj - (ex >> 8);
+    const int ic = ok && m != 1 ? sa[ii] : 0;
+    const int jc = ok && m != 0 ? sb[jj] : 0;
+    int vri = kGap, vrj = kGap, vmw = kGap, vmd = -1;
+    if (m == 0) {
+      vri = ic;
+      vmw = (ic & kGap) ? kGap : ic | kGap;
+      if (!(ic & kGap)) vmd = ic | kGap;
+    } else if (m == 1) {
+      vrj = jc;
+      vmw = (jc & kGap) ? kGap : jc | kGap;
+      if (!(jc & kGap)) vmd = jc | kGap;
+    } else if (m == 2) {
+      vri = ic;
+      vrj = jc;
+    } else {
+      vri = ic;
+      vrj = jc;
+      vmw = vmd = s_med[((ic & 15) << 5) + (jc & 15)];
+    }
+    const bool emd = ok && vmd >= 0;
+    int mtot;
+    const int mex = wave_excl_sum(emd ? 1 : 0, mtot);
+    if (ok) {
+      ri[cap - 1 - s] = (uint8_t)vri;
+      rj[cap - 1 - s] = (uint8_t)vrj;
+      mw[cap - 1 - s] = (uint8_t)vmw;
+    }
+    if (emd) md[cap - 1 - (cm + mex)] = (uint8_t)vmd;
+    if (mtot > 0) {
+      const int lastl = 63 - __builtin_clzll(__builtin_amdgcn_ballot_w64(emd));
+      mlast = __builtin_amdgcn_readlane(vmd, lastl);
+    }
+    ci += tot & 255;
+    cj += tot >> 8;
+    cm += mtot;
   }
-  while (j != 0) {
-    if (!(jc & kGap)) { PUT(md, nm, jc | kGap); PUT(mw, nw, jc | kGap); }
-    else PUT(mw, nw, kGap);
-    PUT(ri, ni, kGap);
-    PUT(rj, nj, jc);
-    j--;
-    jc = sb[j];
+  // the leading gap column; the median takes it unless it already ends so
+  if (lane == 0) {
+    ri[cap - 1 - n] = kGap;
+    rj[cap - 1 - n] = kGap;
+    mw[cap - 1 - n] = kGap;
+    if (cm == 0 || mlast != kGap) md[cap - 1 - cm] = kGap;
   }
-  PUT(ri, ni, kGap);
-  PUT(rj, nj, kGap);
-  PUT(mw, nw, kGap);
-  if (nm == 0 || md[cap - nm] != kGap) PUT(md, nm, kGap);
-#undef PUT
-  A.outlen[4 * pid + (geo ? 1 : 0)] = ni;
-  A.outlen[4 * pid + (geo ? 0 : 1)] = nj;
-  A.outlen[4 * pid + 2] = nm;
-  A.outlen[4 * pid + 3] = nw;
+  if (cm == 0 || mlast != kGap) cm++;
+  if (lane == 0) {
+    A.outlen[4 * pid + (geo ? 1 : 0)] = n + 1;
+    A.outlen[4 * pid + (geo ? 0 : 1)] = n + 1;
+    A.outlen[4 * pid + 2] = cm;
+    A.outlen[4 * pid + 3] = n + 1;
+  }
 }
 
 // algn_CAML_cost_affine_3 (src/algn.c:2457) -> algn_fill_plane_3_aff_nobt

@@ -751,7 +751,7 @@ int batch_enqueue(poyhip_batch* b, const uint8_t* blob, const int64_t* off1, con
       B.res = b->d_res;
       B.out = d_out;
       B.outlen = d_outlen;
-      hipLaunchKernelGGL(k_affine_backtrace, dim3((t.n + 63) / 64), dim3(64), 0, st, B);
+      hipLaunchKernelGGL(k_affine_backtrace, dim3(t.n), dim3(64), 0, st, B);
     } else if (b->op == POYHIP_OP_ALIGN_2D) {
       LinBtArgs B;
       B.pt = t;

@@ -307,7 +307,7 @@ int srv_launch(poyhip_server* s, SrvSlot& sl) {
     B.res = sl.d_res;
     B.out = sl.d_out;
     B.outlen = sl.d_outlen;
-    hipLaunchKernelGGL(k_affine_backtrace, dim3((n + 63) / 64), dim3(64), 0, st, B);
+    hipLaunchKernelGGL(k_affine_backtrace, dim3(n), dim3(64), 0, st, B);
   } else if (op == POYHIP_OP_COST_AFFINE) {
     AffCostArgs A;
     A.pt = pt;
