@@ -208,9 +208,17 @@ __global__ __launch_bounds__(64) void k_affine_classify(PairTable pt, uint8_t* c
 // through DPP; lane 0 takes it from a FIFO in LDS that lane 63 fills one
 // stripe earlier (indexed by column mod kRotFifo).  Every lane loads its own
 // column symbols: lanes of different stripes work on unrelated columns, so
-// the symbol cannot travel down the wavefront as in the plain loop.  A round whose schedule
-// needs a deeper FIFO, or more stripes than kRotMaxStripes, runs the plain
-// stripe loop instead.
+// the symbol cannot travel down the wavefront as in the plain loop.
+//
+// The FIFO holds lag + 1 entries, lag ~ W - 123, so a band wider than about
+// 560 columns outgrows it.  Such a round spills the FIFO to the wavefront's
+// global workspace (one 16-byte entry per column): lane 63 still writes its
+// four values per step to shared memory, lanes 0-3 copy the block's four
+// entries to global memory at the end of the block, and lanes 0-4 load the
+// entries lane 0 needs two blocks ahead into registers and park them in a
+// 16-entry shared ring one block ahead, so the loads' latency hides behind a
+// block of cells.  A round with more stripes than kRotMaxStripes runs the
+// plain stripe loop instead.
 //
 // Direction words are laid out [global step / 4][lane] for the whole round
 // and stab[b] = S_b, so the backtrace finds cell (row i0_b + t, column j) of
@@ -250,12 +258,16 @@ __device__ __forceinline__ int band_hi(int i, int D, int newk, int lenj) {
 //       outputs through DPP one step later, so it still sees lane t's last
 //       cell of stripe b; at its own last cell (the band's right edge) it
 //       ignores the row above.
-// The FIFO is safe while lag = S_{b+1} - S_b - (jlo_{b+1} - jlo_b) - 63
-// stays within kRotFifo - 72 (no entry is overwritten before it is read).
+// The shared FIFO is safe while lag = S_{b+1} - S_b - (jlo_{b+1} - jlo_b)
+// - 63 stays within kRotFifo - 72 (no entry is overwritten before it is
+// read); a larger lag spills it (lag_out).  The spilled FIFO needs lag >= 24:
+// an entry reaches global memory at most 3 steps after lane 63 computes it
+// and is loaded at most 12 steps before lane 0 reads it (C1 gives lag >= 8,
+// the spill starts above 400).
 // Gap counts travel as two u16 halves of one word with 32-bit adds, exact
 // while both lengths stay below 2^16 (the counts are bounded by them).
 __device__ int rot_plan(RotShared& sh, int leni, int lenj, int D, int newk, int nst,
-                        long long dm_steps_cap) {
+                        long long dm_steps_cap, int& lag_out) {
   const int lane = threadIdx.x;
   if (nst > kRotMaxStripes || lenj > 65535) return -1;
   for (int b = lane; b < nst; b += 64) {
@@ -293,9 +305,8 @@ __device__ int rot_plan(RotShared& sh, int leni, int lenj, int D, int newk, int 
   const int total = __builtin_amdgcn_readfirstlane(sh.S[nst]);
   const int lag = __builtin_amdgcn_readfirstlane(sh.lag);
   __syncthreads();
-  if (lag > kRotFifo - 72 || (long long)((total + 3) >> 2) * 4 > dm_steps_cap ||
-      total > (1 << 24))
-    return -1;
+  lag_out = lag;
+  if ((long long)((total + 3) >> 2) * 4 > dm_steps_cap || total > (1 << 24)) return -1;
   return total;
 }
 
@@ -318,6 +329,7 @@ __device__ __forceinline__ int ld_sym(__amdgpu_buffer_rsrc_t r, int off) {
 }
 
 typedef unsigned short rot_u16x2 __attribute__((ext_vector_type(2)));
+typedef unsigned rot_u32x4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ int max_u16x2(int a, int b) {
   return __builtin_bit_cast(int, __builtin_elementwise_max(__builtin_bit_cast(rot_u16x2, a),
                                                             __builtin_bit_cast(rot_u16x2, b)));
@@ -352,17 +364,29 @@ __device__ __forceinline__ int shift_in(int acc, lmask m) {
 // HORIZONTAL_EQ_VERTICAL; the align-mode move of a cell is the todo move of
 // its diagonal predecessor, where the backtrace reads it.  Layout: per 4-step
 // block one 16-bit word per lane, step u of the block in bits 15-4u..12-4u.
-template <bool kSw>
+//
+// kG: the FIFO spilled to gfifo (entries of V, CB, H and gap counts for
+// columns -64..lenj at entry column + 64: lane 63 runs from column -63 before
+// its first row and lane 0 reads column -1 at most, so every offset stays
+// non-negative; columns past lenj fall outside the resource).  Shared-memory
+// slots then: 0-15 the ring of lane 0's
+// entries by step, 16 lane 0's up-left entry at its next stripe, 32-35 lane
+// 63's entries of the current block.
+template <bool kSw, bool kG>
 __device__ void rot_round_fast(RotShared& sh, int total, const uint8_t* __restrict__ sa,
                                const uint8_t* __restrict__ sb, int leni, int lenj, int D, int newk,
                                int nst, int go, int HA0, const int* s_cost, const int* s_ge,
-                               const int* s_prep, uint8_t* dm, long long* stab, int& rc_out,
-                               int& rgn_out, long long& cells, long long& slots) {
+                               const int* s_prep, uint8_t* dm, long long* stab, int* gfifo,
+                               int& rc_out, int& rgn_out, long long& cells, long long& slots) {
   const int lane = threadIdx.x;
   const int kNever = 0x7fffffff;
   for (int b = lane; b <= nst; b += 64) stab[b] = sh.S[b];
   const __amdgpu_buffer_rsrc_t rsym = seq_rsrc(sb, lenj + 1);
   const __amdgpu_buffer_rsrc_t rdm = seq_rsrc(dm, ((total + 3) >> 2) << 7);
+  // stores past column lenj are dropped, loads there return 0
+  const __amdgpu_buffer_rsrc_t rgf =
+      seq_rsrc(reinterpret_cast<const uint8_t*>(gfifo), (lenj + 65) << 4);
+  rot_u32x4 pf = {0, 0, 0, 0};  // kG, lanes 0-4: entries loaded a block ago
   const int jlo_t = band_lo(1, newk);
   // ---- current row: stripe 0 -----------------------------------------------
   int symA = sa[imin(1 + lane, leni)];  // row symbols of even / odd stripes,
@@ -487,7 +511,7 @@ __device__ void rot_round_fast(RotShared& sh, int total, const uint8_t* __restri
           // lanes keep theirs
           int4 v = make_int4(kHigh, kHigh, kHigh, 0);
           if (jlo0 >= 1) {
-            const int x = (jlo0 - 1) & (kRotFifo - 1);
+            const int x = kG ? 16 : (jlo0 - 1) & (kRotFifo - 1);
             v = make_int4(sh.fV[x], sh.fC[x], sh.fH[x], sh.fG[x]);
           }
           if (lane == 0) {
@@ -547,7 +571,7 @@ __device__ void rot_round_fast(RotShared& sh, int total, const uint8_t* __restri
       // lane 63 hands its row to lane 0 of the next stripe (columns outside
       // its band land in slots that are rewritten before they are read)
       if (lane == 63) {
-        const int x = (st + c63) & (kRotFifo - 1);
+        const int x = kG ? 32 + u : (st + c63) & (kRotFifo - 1);
         sh.fV[x] = V;
         sh.fC[x] = C;
         sh.fH[x] = H;
@@ -568,7 +592,7 @@ __device__ void rot_round_fast(RotShared& sh, int total, const uint8_t* __restri
     auto read_fifo = [&]() {
 #pragma unroll
       for (int u = 0; u < 4; u++) {
-        const int x = col0(s + u) & (kRotFifo - 1);
+        const int x = kG ? (s + u) & 15 : col0(s + u) & (kRotFifo - 1);
         fvx[u] = sh.fV[x];
         fvy[u] = sh.fC[x];
         fvz[u] = sh.fH[x];
@@ -588,6 +612,30 @@ __device__ void rot_round_fast(RotShared& sh, int total, const uint8_t* __restri
       step(2, std::true_type{}); step(3, std::true_type{});
     }
     __builtin_amdgcn_raw_buffer_store_b16((unsigned short)pack, rdm, lane << 1, (s >> 2) << 7, 0);
+    if constexpr (kG) {
+      // lane 63's writes of this block precede the copies below
+      __builtin_amdgcn_wave_barrier();
+      if (lane < 5) {
+        // lanes 0-3: this block's entries of lane 63 (column s + u + c63) to
+        // global memory
+        if (lane < 4) {
+          const rot_u32x4 v = {(unsigned)sh.fV[32 + lane], (unsigned)sh.fC[32 + lane],
+                               (unsigned)sh.fH[32 + lane], (unsigned)sh.fG[32 + lane]};
+          __builtin_amdgcn_raw_buffer_store_b128(v, rgf, (s + lane + c63 + 64) << 4, 0, 0);
+        }
+        // the entries loaded at the end of the previous block: lane 0's
+        // steps s + 4..s + 7, and lane 4 its up-left entry at the next stripe
+        const int x = lane < 4 ? (s + 4 + lane) & 15 : 16;
+        sh.fV[x] = (int)pf.x;
+        sh.fC[x] = (int)pf.y;
+        sh.fH[x] = (int)pf.z;
+        sh.fG[x] = (int)pf.w;
+        // load steps s + 8..s + 11 (at most one stripe change in 12 steps,
+        // so col0 holds), device scope: lane 63's stores went to L2
+        const int col = lane < 4 ? col0(s + 8 + lane) : jlon0 - 1;
+        pf = __builtin_amdgcn_raw_buffer_load_b128(rgf, (col + 64) << 4, 0, 16);
+      }
+    }
   }
   __builtin_amdgcn_wave_barrier();
   const int2 fin = sh.fin;
@@ -695,14 +743,31 @@ __global__ __launch_bounds__(64, 4) void k_affine_fill(AffFillArgs A) {
       int rc = 0, rgn = 0;
       int rot_total = -1;
       if constexpr (kFast && kDm) {
-        rot_total = rot_plan(s_rot, leni, lenj, D, newk, nstripes, aff_dm_bytes(La, Lb) >> 5);
+        int lag = 0;
+        rot_total =
+            rot_plan(s_rot, leni, lenj, D, newk, nstripes, aff_dm_bytes(La, Lb) >> 5, lag);
         if (rot_total > 0) {
-          if (sw)
-            rot_round_fast<true>(s_rot, rot_total, sa, sb, leni, lenj, D, newk, nstripes, go, HA0,
-                                 s_cost, s_ge, s_prep, dm, stab, rc, rgn, cells, slots);
-          else
-            rot_round_fast<false>(s_rot, rot_total, sa, sb, leni, lenj, D, newk, nstripes, go, HA0,
-                                  s_cost, s_ge, s_prep, dm, stab, rc, rgn, cells, slots);
+          // the global FIFO (Lb + 64 entries of 4 ints) reuses the stripe
+          // loop's row buffers (16 * Lb + 16 ints; the spill needs lenj > 400)
+          if (lag > kRotFifo - 72) {
+            if (sw)
+              rot_round_fast<true, true>(s_rot, rot_total, sa, sb, leni, lenj, D, newk, nstripes,
+                                         go, HA0, s_cost, s_ge, s_prep, dm, stab, bnd, rc, rgn,
+                                         cells, slots);
+            else
+              rot_round_fast<false, true>(s_rot, rot_total, sa, sb, leni, lenj, D, newk, nstripes,
+                                          go, HA0, s_cost, s_ge, s_prep, dm, stab, bnd, rc, rgn,
+                                          cells, slots);
+          } else {
+            if (sw)
+              rot_round_fast<true, false>(s_rot, rot_total, sa, sb, leni, lenj, D, newk, nstripes,
+                                          go, HA0, s_cost, s_ge, s_prep, dm, stab, bnd, rc, rgn,
+                                          cells, slots);
+            else
+              rot_round_fast<false, false>(s_rot, rot_total, sa, sb, leni, lenj, D, newk,
+                                           nstripes, go, HA0, s_cost, s_ge, s_prep, dm, stab, bnd,
+                                           rc, rgn, cells, slots);
+          }
         }
       }
       for (int b = 0; rot_total <= 0 && b < nstripes; b++) {

@@ -168,6 +168,45 @@ def test_batch_affine_config2_long(ctx):
         assert np.array_equal(outs[k][3], ref["medianwg"]), k
 
 
+def wide_band_pairs(seed):
+    """Gap-free pairs whose doubling rounds reach bands of 600-1400 columns:
+    long-vs-short random pairs (one wide round) and block-indel pairs whose
+    narrow round is followed by a wide one, in both orders."""
+    rng = np.random.default_rng(seed)
+    pairs = []
+    for la, lb in [(200, 800), (130, 700), (66, 650), (300, 950), (257, 1000), (129, 720),
+                   (400, 1100), (640, 1300)]:
+        a = np.concatenate([[16], rng.choice(PLAIN, la - 1)]).astype(np.uint8)
+        b = np.concatenate([[16], rng.choice(PLAIN, lb - 1)]).astype(np.uint8)
+        pairs.append((a, b))
+    for la, ins_at, x, del_at, y in [(800, 200, 200, 500, 100), (1000, 300, 230, 700, 130),
+                                     (700, 50, 300, 600, 200)]:
+        a = np.concatenate([[16], rng.choice(PLAIN, la - 1)]).astype(np.uint8)
+        b = np.concatenate([a[:ins_at], rng.choice(PLAIN, x), a[ins_at:del_at],
+                            a[del_at + y:]]).astype(np.uint8)
+        pairs.append((a, b))
+    return pairs + [(b, a) for a, b in pairs[::3]]
+
+
+def test_batch_affine_wide_bands(ctx):
+    """Bands wider than the shared-memory FIFO of the rotating schedule: the
+    fill spills it to global memory."""
+    say("test_batch_affine_wide_bands")
+    ctx.set_cm(cost_matrix.of_transformations_and_gaps(1, 1, 3))
+    ocm = oracle.cm_dna(1, 1, 3)
+    pairs = wide_band_pairs(0x71DE)
+    costs, outs = run_pairs(ctx, "align_affine", pairs)
+    for k, (a, b) in enumerate(pairs):
+        sw = len(a) > len(b)
+        si, sj = (b, a) if sw else (a, b)
+        ref = oracle.align_affine_3(si, sj, ocm, int(sw))
+        ra, rb = (ref["resj"], ref["resi"]) if sw else (ref["resi"], ref["resj"])
+        assert costs[k] == ref["cost"], (k, costs[k], ref["cost"])
+        assert np.array_equal(outs[k][0], ra) and np.array_equal(outs[k][1], rb), k
+        assert np.array_equal(outs[k][2], ref["median"]), k
+        assert np.array_equal(outs[k][3], ref["medianwg"]), k
+
+
 def test_batch_linear_config1_sample(ctx):
     say("test_batch_linear_config1_sample")
     ctx.set_cm(cost_matrix.of_transformations_and_gaps(1, 2))
