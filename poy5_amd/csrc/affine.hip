@@ -228,9 +228,10 @@ constexpr int kRotFifo = 512;        // entries (power of two)
 constexpr int kRotMaxStripes = 128;
 
 struct RotShared {
-  // V, CB, H and gap counts of row i0_b - 1, one array each (lane 0 reads
-  // each component into its own register)
-  int fV[kRotFifo], fC[kRotFifo], fH[kRotFifo], fG[kRotFifo];
+  // V, CB + gap opening, the best state f and the horizontal gap count of
+  // row i0_b - 1, one array each (lane 0 reads each component into its own
+  // register)
+  int fV[kRotFifo], fC[kRotFifo], fF[kRotFifo], fG[kRotFifo];
   int S[kRotMaxStripes + 1];         // stripe start steps
   int lag;                           // largest FIFO lag of the round
   int2 fin;                          // cost and gap counts of the last cell
@@ -264,8 +265,8 @@ __device__ __forceinline__ int band_hi(int i, int D, int newk, int lenj) {
 // an entry reaches global memory at most 3 steps after lane 63 computes it
 // and is loaded at most 12 steps before lane 0 reads it (C1 gives lag >= 8,
 // the spill starts above 400).
-// Gap counts travel as two u16 halves of one word with 32-bit adds, exact
-// while both lengths stay below 2^16 (the counts are bounded by them).
+// The gap counts are POY's u16 rows (DIRECTION_MATRIX); the rotating fill
+// keeps the horizontal one in 32 bits, equal while lenj stays below 2^16.
 __device__ int rot_plan(RotShared& sh, int leni, int lenj, int D, int newk, int nst,
                         long long dm_steps_cap, int& lag_out) {
   const int lane = threadIdx.x;
@@ -365,13 +366,12 @@ __device__ __forceinline__ int shift_in(int acc, lmask m) {
 // its diagonal predecessor, where the backtrace reads it.  Layout: per 4-step
 // block one 16-bit word per lane, step u of the block in bits 15-4u..12-4u.
 //
-// kG: the FIFO spilled to gfifo (entries of V, CB, H and gap counts for
-// columns -64..lenj at entry column + 64: lane 63 runs from column -63 before
-// its first row and lane 0 reads column -1 at most, so every offset stays
-// non-negative; columns past lenj fall outside the resource).  Shared-memory
-// slots then: 0-15 the ring of lane 0's
-// entries by step, 16 lane 0's up-left entry at its next stripe, 32-35 lane
-// 63's entries of the current block.
+// kG: the FIFO spilled to gfifo, the four components of columns -64..lenj
+// at entry column + 64 (lane 63 runs from column -63 before its first row
+// and lane 0 reads column -1 at most, so every offset stays non-negative;
+// columns past lenj fall outside the resource).  Shared-memory slots then:
+// 0-15 the ring of lane 0's entries by step, 16 lane 0's up-left entry at
+// its next stripe, 32-35 lane 63's entries of the current block.
 template <bool kSw, bool kG>
 __device__ void rot_round_fast(RotShared& sh, int total, const uint8_t* __restrict__ sa,
                                const uint8_t* __restrict__ sb, int leni, int lenj, int D, int newk,
@@ -416,12 +416,9 @@ __device__ void rot_round_fast(RotShared& sh, int total, const uint8_t* __restri
                     (lenj - band_lo(1 + (blast << 6), newk)) + ((leni - 1) & 63);
   const int src = (leni - 1) & 63;  // lane of the last row
   // up-left cell of lane 0's first step: row 0 at column jlo0 - 1
-  int dV = kHigh, dCB = kHigh, dH = kHigh, dG = 0;
-  if (jlo0 >= 1) {
-    dCB = 0;
-    dH = HA0;
-  }
-  int oV = kHigh, oCB = kHigh, oH = kHigh, oG = 0;
+  // (its best state: CB 0, H HA0, V HIGH_NUM)
+  int dF = jlo0 >= 1 ? imin(0, HA0) : kHigh, dG = 0;
+  int oV = kHigh, oCBg = kHigh + go, oH = kHigh, oF = kHigh, oG = 0;
   int cb0run = 0;
   int c0prev = sb[imax(jlo0 - 1, 0)];
   slots += (long long)total * 64;
@@ -509,18 +506,19 @@ __device__ void rot_round_fast(RotShared& sh, int total, const uint8_t* __restri
           jlon0 = band_lo(1 + ((b0 + 1) << 6), newk);
           // lane 0's up-left cell: row i0 - 1 at column jlo0 - 1; the other
           // lanes keep theirs
-          int4 v = make_int4(kHigh, kHigh, kHigh, 0);
+          int2 v = make_int2(kHigh, 0);
           if (jlo0 >= 1) {
             const int x = kG ? 16 : (jlo0 - 1) & (kRotFifo - 1);
-            v = make_int4(sh.fV[x], sh.fC[x], sh.fH[x], sh.fG[x]);
+            v = make_int2(sh.fF[x], sh.fG[x]);
           }
           if (lane == 0) {
-            dV = v.x; dCB = v.y; dH = v.z; dG = v.w;
+            dF = v.x;
+            dG = v.y;
           }
         }
       }
       // lane 0's source row: POY's row 0 in stripe 0, else the FIFO
-      int sV = fvx[u], sCB = fvy[u], sH = fvz[u], sG = fvw[u];
+      int sV = fvx[u], sCBg = fvy[u], sF = fvz[u], sG = fvw[u];
       if constexpr (kSlow) {
         const int c0 = c0b[u];
         if (b0 == 0) {
@@ -529,43 +527,45 @@ __device__ void rot_round_fast(RotShared& sh, int total, const uint8_t* __restri
           const int gop0 = has_gap_opening(jl0, c0prev, c0, go);
           const int add0 = ((c0prev & kGap) && !(c0 & kGap)) ? gop0 + gr0 : gr0;
           cb0run = (jl0 == 0) ? 0 : cb0run + add0;
+          // V HIGH_NUM, CB cb0run, H HA0 at column 0 and cb0run after it
           sV = kHigh;
-          sCB = cb0run;
-          sH = (jl0 == 0) ? HA0 : cb0run;
-          sG = jl0 & 0xFFFF;
+          sCBg = cb0run + go;
+          sF = (jl0 == 0) ? imin(0, HA0) : cb0run;
+          sG = jl0;
         }
         c0prev = c0;
       }
       const int uV = wave_shr1(oV, sV);
-      const int uCB = wave_shr1(oCB, sCB);
-      const int uH = wave_shr1(oH, sH);
+      const int uCBg = wave_shr1(oCBg, sCBg);
+      const int uF = wave_shr1(oF, sF);
       const int uG = wave_shr1(oG, sG);
-      // the cell (aff_cell_fast_core with packed gap counts); tie and end
-      // flags are lane masks, combined on the scalar unit.  Flags the
-      // backtrace never reads are left unmasked: column 0 (the walk stops
-      // there), END_VERTICAL at the right edge and END_HORIZONTAL at the
-      // left edge (a run reaching them would extend a HIGH_NUM state).
+      // the cell (aff_cell_fast_core); tie and end flags are lane masks,
+      // combined on the scalar unit.  Flags the backtrace never reads are
+      // left unmasked: column 0 (the walk stops there), END_VERTICAL at the
+      // right edge and END_HORIZONTAL at the left edge (a run reaching them
+      // would extend a HIGH_NUM state).
+      //   Each lane passes down CB + gap opening (both uses add it) and the
+      // cell's best state f, the diagonal's min(CB, V, H) one step later.
       const int gr = grb[u], d = cdb[u];
       const bool at_left = st == fst, at_right = st == rs;
       const bool j0 = kSlow && st == zs;  // quiet blocks hold no column-0 cell
-      const int hb = oCB + go;
-      const int H = at_left ? kHigh : gr + imin(oH, hb);
-      const int vb = uCB + go;
-      const int V = at_right ? kHigh : ge + imin(uV, vb);
-      const int m3 = imin(imin(dCB, dV), dH);
-      const int C = j0 ? kHigh : m3 + d;
+      const int H = at_left ? kHigh : gr + imin(oH, oCBg);
+      const int V = at_right ? kHigh : ge + imin(uV, uCBg);
+      const int C = j0 ? kHigh : dF + d;
+      const int CBg = C + go;
       const int f = imin(imin(H, V), C);
       const bool dh = H == f, dv = V == f, da = C == f;
       const lmask mdh = ballot(dh), mdv = ballot(dv);
       const lmask mheqv = mdh & ballot(H == V);
-      // algn_fill_gapnum on both u16 halves at once: horizontal adds 1 to
-      // the low half, vertical to the high half
-      const int G = max_u16x2(max_u16x2(da ? dG : 0, dh ? oG + 1 : 0), dv ? uG + 0x10000 : 0);
+      // algn_fill_gapnum: horizontal moves over the tied predecessors.  Every
+      // path into (i, j) has j - i more horizontal than vertical moves, so
+      // the largest horizontal count also has the largest vertical one.
+      const int G = imax(imax(da ? dG : 0, dh ? oG + 1 : 0), dv ? uG : 0);
       // direction nibble, high bit first: horizontal end, vertical end,
       // then choose_dir of the todo mode (bit 1: neither vertical nor
       // horizontal; bit 0: not vertical, or both with the swap flag)
-      pack = shift_in(pack, ballot(oH >= hb) | mheqv);
-      pack = shift_in(pack, ballot(uV >= vb) | mheqv);
+      pack = shift_in(pack, ballot(oH >= oCBg) | mheqv);
+      pack = shift_in(pack, ballot(uV >= uCBg) | mheqv);
       pack = shift_in(pack, ~(mdv | mdh));
       pack = shift_in(pack, kSw ? ~mdv | mdh : ~mdv);
       // lane 63 hands its row to lane 0 of the next stripe (columns outside
@@ -573,18 +573,18 @@ __device__ void rot_round_fast(RotShared& sh, int total, const uint8_t* __restri
       if (lane == 63) {
         const int x = kG ? 32 + u : (st + c63) & (kRotFifo - 1);
         sh.fV[x] = V;
-        sh.fC[x] = C;
-        sh.fH[x] = H;
+        sh.fC[x] = CBg;
+        sh.fF[x] = f;
         sh.fG[x] = G;
       }
-      // the last cell of the round: cost and gap counts
+      // the last cell of the round: cost and horizontal gap count
       if constexpr (kSlow) {
         if (st == s_fin) {
           if (lane == src) sh.fin = make_int2(f, G);
         }
       }
-      oV = V; oCB = C; oH = H; oG = G;
-      dV = uV; dCB = uCB; dH = uH; dG = uG;
+      oV = V; oCBg = CBg; oH = H; oF = f; oG = G;
+      dF = uF; dG = uG;
     };
     // lane 0's FIFO entries of the block, read at the start of each form:
     // one register per component, each the DPP destination of its up value
@@ -595,7 +595,7 @@ __device__ void rot_round_fast(RotShared& sh, int total, const uint8_t* __restri
         const int x = kG ? (s + u) & 15 : col0(s + u) & (kRotFifo - 1);
         fvx[u] = sh.fV[x];
         fvy[u] = sh.fC[x];
-        fvz[u] = sh.fH[x];
+        fvz[u] = sh.fF[x];
         fvw[u] = sh.fG[x];
       }
     };
@@ -620,7 +620,7 @@ __device__ void rot_round_fast(RotShared& sh, int total, const uint8_t* __restri
         // global memory
         if (lane < 4) {
           const rot_u32x4 v = {(unsigned)sh.fV[32 + lane], (unsigned)sh.fC[32 + lane],
-                               (unsigned)sh.fH[32 + lane], (unsigned)sh.fG[32 + lane]};
+                               (unsigned)sh.fF[32 + lane], (unsigned)sh.fG[32 + lane]};
           __builtin_amdgcn_raw_buffer_store_b128(v, rgf, (s + lane + c63 + 64) << 4, 0, 0);
         }
         // the entries loaded at the end of the previous block: lane 0's
@@ -628,7 +628,7 @@ __device__ void rot_round_fast(RotShared& sh, int total, const uint8_t* __restri
         const int x = lane < 4 ? (s + 4 + lane) & 15 : 16;
         sh.fV[x] = (int)pf.x;
         sh.fC[x] = (int)pf.y;
-        sh.fH[x] = (int)pf.z;
+        sh.fF[x] = (int)pf.z;
         sh.fG[x] = (int)pf.w;
         // load steps s + 8..s + 11 (at most one stripe change in 12 steps,
         // so col0 holds), device scope: lane 63's stores went to L2
@@ -640,7 +640,8 @@ __device__ void rot_round_fast(RotShared& sh, int total, const uint8_t* __restri
   __builtin_amdgcn_wave_barrier();
   const int2 fin = sh.fin;
   rc_out = fin.x;
-  rgn_out = imax(fin.y & 0xFFFF, (int)((unsigned)fin.y >> 16));
+  // the last cell (leni, lenj) has lenj - leni >= 0 more horizontal moves
+  rgn_out = fin.y;
 }
 
 // At most 128 VGPRs: four wavefronts per SIMD (shared memory allows as many)
