@@ -30,7 +30,31 @@ struct AffFillArgs {
   int* counter;
   const uint8_t* cls;  // per-pair class (1: gap-free fast path), or null
   const int* ngen;     // number of general-class pairs, or null
+  // output blocks and lengths (as AffBtArgs): when set, the gap-free
+  // kernel's wavefronts run each pair's backtrace right after its fill
+  uint8_t* out = nullptr;
+  int* outlen = nullptr;
 };
+
+__device__ __forceinline__ void aff_backtrace_pair(PairTable pt, const uint8_t* dm_all,
+                                                   const long long* stab_all, uint8_t* out,
+                                                   int* outlen, const uint8_t* s_med, int pid,
+                                                   int newk, bool nib);
+
+// The backtrace of pair pid right after its fill, by the same wavefront; the
+// median table goes into s_med (shared memory idle between pairs).
+__device__ __forceinline__ void aff_fused_backtrace(PairTable pt, const CmDev* cm,
+                                                    const uint8_t* dm_all,
+                                                    const long long* stab_all, uint8_t* out,
+                                                    int* outlen, uint8_t* s_med, int pid,
+                                                    int newk, bool nib) {
+  // the direction store's words come from this wavefront's lanes
+  __threadfence_block();
+  for (int k = threadIdx.x; k < 512; k += 64) s_med[k] = cm->median[k];
+  __syncthreads();
+  aff_backtrace_pair(pt, dm_all, stab_all, out, outlen, s_med, pid, newk, nib);
+  __syncthreads();
+}
 
 // Reads `p[j]` for j in [0, n); other j read p[0] (the value is unused).
 __device__ __forceinline__ int ldc(const uint8_t* p, int j, int n) {
@@ -711,6 +735,11 @@ __global__ __launch_bounds__(64, 4) void k_affine_fill(AffFillArgs A) {
         r.slots = 0;
         A.res[pid] = r;
       }
+      if constexpr (kDm && kFast) {
+        if (A.out != nullptr)
+          aff_fused_backtrace(A.pt, A.cm, A.dm, A.stab, A.out, A.outlen,
+                              reinterpret_cast<uint8_t*>(s_rot.fV), pid, -1, false);
+      }
       continue;
     }
 
@@ -912,6 +941,12 @@ __global__ __launch_bounds__(64, 4) void k_affine_fill(AffFillArgs A) {
       r.slots = slots;
       A.res[pid] = r;
     }
+    if constexpr (kDm && kFast) {
+      // its median table in the rotating schedule's FIFO
+      if (A.out != nullptr)
+        aff_fused_backtrace(A.pt, A.cm, A.dm, A.stab, A.out, A.outlen,
+                            reinterpret_cast<uint8_t*>(s_rot.fV), pid, newk, kFast);
+    }
   }
 }
 
@@ -923,6 +958,10 @@ struct AffBtArgs {
   const PairResult* res;
   uint8_t* out;
   int* outlen;  // 4 per pair: aligned length, aligned length, median, median-with-gaps
+  // when set: only the general-class pairs (cls 0; the gap-free kernel ran
+  // the others' backtraces), none when the count ngen is 0
+  const uint8_t* cls = nullptr;
+  const int* ngen = nullptr;
 };
 
 // This lane's direction word in global 4-step block q of a pair's store
@@ -969,33 +1008,30 @@ __device__ __forceinline__ int wave_excl_sum(int x, int& total) {
 // the median only where it keeps a column, compacted by a second prefix
 // sum).  Outputs are written right to left into four regions of capacity
 // La+Lb+2 (resi, resj, median, medianwg).
-__global__ __launch_bounds__(64) void k_affine_backtrace(AffBtArgs A) {
-  // median table rows 0..15 in shared memory
-  __shared__ uint8_t s_med[512];
+// Pair pid of the table; newk and the direction format (nib: gap-free
+// nibbles) as its fill left them; s_med: median table rows 0..15.
+__device__ __forceinline__ void aff_backtrace_pair(PairTable pt, const uint8_t* dm_all,
+                                                   const long long* stab_all, uint8_t* out,
+                                                   int* outlen, const uint8_t* s_med, int pid,
+                                                   int newk, bool nib) {
   const int lane = threadIdx.x;
-  for (int k = lane; k < 512; k += 64) s_med[k] = A.cm->median[k];
-  __syncthreads();
-  const int pid = __builtin_amdgcn_readfirstlane(A.pt.order[blockIdx.x]);
-  const uint8_t* sa = A.pt.blob + rows_off(A.pt, pid);
-  const uint8_t* sb = A.pt.blob + cols_off(A.pt, pid);
-  const int La = A.pt.len_a[pid], Lb = A.pt.len_b[pid];
-  const int geo = A.pt.swapped[pid];
+  const uint8_t* sa = pt.blob + rows_off(pt, pid);
+  const uint8_t* sb = pt.blob + cols_off(pt, pid);
+  const int La = pt.len_a[pid], Lb = pt.len_b[pid];
+  const int geo = pt.swapped[pid];
   const int cap = La + Lb + 2;
-  uint8_t* blk = A.out + A.pt.out_off[pid];
+  uint8_t* blk = out + pt.out_off[pid];
   // regions in the caller's order: first sequence, second sequence
   uint8_t* ri = geo ? blk + cap : blk;
   uint8_t* rj = geo ? blk : blk + cap;
   uint8_t* md = blk + 2 * cap;
   uint8_t* mw = blk + 3 * cap;
-  const uint8_t* dm = A.dm + A.pt.dm_off[pid];
-  const long long* stab = A.stab + A.pt.stab_off[pid];
-  const PairResult R = A.res[pid];
-  const int newk = R.newk;
+  const uint8_t* dm = dm_all + pt.dm_off[pid];
+  const long long* stab = stab_all + pt.stab_off[pid];
   const long long dm_cap = aff_dm_bytes(La, Lb);
   // direction format: bytes (see the top of this file) or, for gap-free
   // pairs, nibbles (todo move, vertical end, horizontal end) whose align-mode
   // move is the todo move of the diagonal predecessor
-  const bool nib = R.full != 0;
   const int vend = nib ? 4 : 16, hend = nib ? 8 : 32;
 
   // ---- walk ----------------------------------------------------------------
@@ -1130,11 +1166,24 @@ __global__ __launch_bounds__(64) void k_affine_backtrace(AffBtArgs A) {
   }
   if (cm == 0 || mlast != kGap) cm++;
   if (lane == 0) {
-    A.outlen[4 * pid + (geo ? 1 : 0)] = n + 1;
-    A.outlen[4 * pid + (geo ? 0 : 1)] = n + 1;
-    A.outlen[4 * pid + 2] = cm;
-    A.outlen[4 * pid + 3] = n + 1;
+    outlen[4 * pid + (geo ? 1 : 0)] = n + 1;
+    outlen[4 * pid + (geo ? 0 : 1)] = n + 1;
+    outlen[4 * pid + 2] = cm;
+    outlen[4 * pid + 3] = n + 1;
   }
+}
+
+__global__ __launch_bounds__(64) void k_affine_backtrace(AffBtArgs A) {
+  // median table rows 0..15 in shared memory
+  __shared__ uint8_t s_med[512];
+  const int lane = threadIdx.x;
+  if (A.ngen != nullptr && __builtin_amdgcn_readfirstlane(*A.ngen) == 0) return;
+  const int pid = __builtin_amdgcn_readfirstlane(A.pt.order[blockIdx.x]);
+  if (A.cls != nullptr && A.cls[pid] != 0) return;
+  for (int k = lane; k < 512; k += 64) s_med[k] = A.cm->median[k];
+  __syncthreads();
+  const PairResult R = A.res[pid];
+  aff_backtrace_pair(A.pt, A.dm, A.stab, A.out, A.outlen, s_med, pid, R.newk, R.full != 0);
 }
 
 // algn_CAML_cost_affine_3 (src/algn.c:2457) -> algn_fill_plane_3_aff_nobt

@@ -716,6 +716,8 @@ int batch_enqueue(poyhip_batch* b, const uint8_t* blob, const int64_t* off1, con
           A.res = b->d_res;
           A.cls = b->d_cls;
           A.ngen = b->d_counters + 2 * nch;
+          A.out = d_out;  // each pair's backtrace follows its fill
+          A.outlen = d_outlen;
           A.counter = b->d_counters + 2 * k;
           hipLaunchKernelGGL((k_affine_fill<true, true>), dim3(b->grid), dim3(64), 0, st, A);
           A.counter = b->d_counters + 2 * k + 1;
@@ -743,6 +745,8 @@ int batch_enqueue(poyhip_batch* b, const uint8_t* blob, const int64_t* off1, con
     HIPCHK(c, hipGetLastError());
     if (timing) HIPCHK(c, hipEventRecord(b->ev[2 * k + 1], st));
     if (b->op == POYHIP_OP_ALIGN_AFFINE) {
+      // the general-class pairs (none in most batches: the kernel then ends
+      // at once); the gap-free kernel ran the other backtraces
       AffBtArgs B;
       B.pt = t;
       B.cm = c->d_cm;
@@ -751,6 +755,8 @@ int batch_enqueue(poyhip_batch* b, const uint8_t* blob, const int64_t* off1, con
       B.res = b->d_res;
       B.out = d_out;
       B.outlen = d_outlen;
+      B.cls = b->d_cls;
+      B.ngen = b->d_counters + 2 * nch;
       hipLaunchKernelGGL(k_affine_backtrace, dim3(t.n), dim3(64), 0, st, B);
     } else if (b->op == POYHIP_OP_ALIGN_2D) {
       LinBtArgs B;

@@ -207,6 +207,28 @@ def test_batch_affine_wide_bands(ctx):
         assert np.array_equal(outs[k][3], ref["medianwg"]), k
 
 
+def test_batch_affine_mixed_classes(ctx):
+    """One batch of gap-free pairs and pairs with gap-bit symbols: the fast
+    kernel fills and backtraces the first class, the general kernel and the
+    backtrace kernel the second; a small direction budget splits the batch
+    into several chunks."""
+    say("test_batch_affine_mixed_classes")
+    ctx.set_cm(cost_matrix.of_transformations_and_gaps(1, 1, 3))
+    ocm = oracle.cm_dna(1, 1, 3)
+    pairs = pair_cases(0xC1A5, 24, 160) + pair_cases(0xC1A6, 24, 160, gappy=0.15)
+    for budget in (0, 1 << 18):
+        costs, outs = run_pairs(ctx, "align_affine", pairs, dm_budget=budget)
+        for k, (a, b) in enumerate(pairs):
+            sw = len(a) > len(b)
+            si, sj = (b, a) if sw else (a, b)
+            ref = oracle.align_affine_3(si, sj, ocm, int(sw))
+            ra, rb = (ref["resj"], ref["resi"]) if sw else (ref["resi"], ref["resj"])
+            assert costs[k] == ref["cost"], (budget, k, costs[k], ref["cost"])
+            assert np.array_equal(outs[k][0], ra) and np.array_equal(outs[k][1], rb), (budget, k)
+            assert np.array_equal(outs[k][2], ref["median"]), (budget, k)
+            assert np.array_equal(outs[k][3], ref["medianwg"]), (budget, k)
+
+
 def test_batch_linear_config1_sample(ctx):
     say("test_batch_linear_config1_sample")
     ctx.set_cm(cost_matrix.of_transformations_and_gaps(1, 2))
