@@ -307,8 +307,9 @@ def main():
     assert np.array_equal(cost.cpu().numpy(), gpu_costs)
 
     # dominant kernel: the fill stage (gap-free and general fill kernels,
-    # one launch of each per chunk); ops are algorithmic: OPS_PER_CELL per
-    # band cell the reference's loops evaluate
+    # one launch of each per chunk; the gap-free kernel also runs each pair's
+    # backtrace after its fill); ops are algorithmic: OPS_PER_CELL per band
+    # cell the reference's loops evaluate, backtrace included
     roofline = None
     if not args.graph:
         fill_ms = float(np.mean(fill_ms_steps))
@@ -319,7 +320,8 @@ def main():
         roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": VALU_PEAK_TOPS,
                     "unit": "TOP/s", "frac": round(achieved / VALU_PEAK_TOPS, 4),
                     "traffic": traffic.get("bytes_per_launch") if traffic else None,
-                    "kernel": "k_affine_fill<true,true> + k_affine_fill<true,false> (fill stage)",
+                    "kernel": "k_affine_fill<true,true> (fill + backtrace) + k_affine_fill<true,false> "
+                              "(fill stage)",
                     "ops_per_cell": OPS_PER_CELL, "launches_per_step": nchunks,
                     "cells_per_launch": cells_launch, "avg_launch_ms": round(launch_ms, 3),
                     "timing": "HIP events on the library stream, every timed step"}
