@@ -964,30 +964,6 @@ struct AffBtArgs {
   const int* ngen = nullptr;
 };
 
-// This lane's direction word in global 4-step block q of a pair's store
-// (bytes: 32-bit words, nibbles: 16-bit words); 0 outside the store.
-__device__ __forceinline__ uint32_t dm_word(const uint8_t* dm, long long cap, bool nib,
-                                            long long q) {
-  const int wlog = nib ? 1 : 2;
-  const long long at = (q << (6 + wlog)) + ((long long)threadIdx.x << wlog);
-  if (q < 0 || at + (1 << wlog) > cap) return 0u;
-  return nib ? (uint32_t)*reinterpret_cast<const uint16_t*>(dm + at)
-             : *reinterpret_cast<const uint32_t*>(dm + at);
-}
-
-// Exclusive prefix sum over the 64 lanes (Hillis-Steele with shuffles) and
-// the total.
-__device__ __forceinline__ int wave_excl_sum(int x, int& total) {
-  int v = x;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const int y = __shfl_up(v, d);
-    if ((int)threadIdx.x >= d) v += y;
-  }
-  total = __builtin_amdgcn_readlane(v, 63);
-  return v - x;
-}
-
 // backtrace_aff (src/algn.c:1715-1819) with follow_*_affine (1531-1592),
 // one wavefront per pair, in two passes.
 //

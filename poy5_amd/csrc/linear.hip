@@ -284,10 +284,24 @@ struct LinBtArgs {
 // pair together with algn_ancestor_2 (3603) for the linear model and
 // seq_get_median_2d_with_gaps (src/seq.c:241).  r1 belongs to the DP rows'
 // sequence; medians are looked up in the caller's argument order.
+//
+// One wavefront per pair, in two passes.  Walk: uniform; the move of a cell
+// is its 2-bit code (0 diagonal, 1 up, 2 left), read from a register window
+// of 4 consecutive 16-step blocks (block q: lane t holds the word of row t)
+// while the next 4 load behind it; every move lowers the cell's global step
+// (within a stripe and into the stripe above), so the windows only move
+// down.  Each step records its move at output position cap-1-step of r1, 64
+// steps per store; the walk ends with the (0, 0) cell, which emits the two
+// leading gaps.  Expansion: 64 steps per pass over the lanes; the row and
+// column of step s are the start minus the moves before it (a prefix sum),
+// the ancestor keeps the steps whose median is not a gap (a second prefix
+// sum).
 __global__ __launch_bounds__(64) void k_linear_backtrace(LinBtArgs A) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= A.pt.n) return;
-  const int pid = A.pt.order[q];
+  __shared__ uint8_t s_med[1024];  // median table, 32 x 32
+  const int lane = threadIdx.x;
+  for (int k = lane; k < 1024; k += 64) s_med[k] = A.cm->median[k];
+  __syncthreads();
+  const int pid = __builtin_amdgcn_readfirstlane(A.pt.order[blockIdx.x]);
   const uint8_t* sa = A.pt.blob + rows_off(A.pt, pid);
   const uint8_t* sb = A.pt.blob + cols_off(A.pt, pid);
   const int L1 = A.pt.len_a[pid], L2 = A.pt.len_b[pid];
@@ -298,61 +312,114 @@ __global__ __launch_bounds__(64) void k_linear_backtrace(LinBtArgs A) {
   uint8_t* r2 = sw ? blk : blk + cap;
   uint8_t* an = blk + 2 * cap;
   uint8_t* mw = blk + 3 * cap;
-  int n1 = 0, n2 = 0, na = 0, nw = 0;
   const uint8_t* dm = A.dm + A.pt.dm_off[pid];
   const long long* stab = A.stab + A.pt.stab_off[pid];
   const PairResult R = A.res[pid];
   const bool full = R.full != 0;
   const int newk = R.newk;
   const long long dm_cap = lin_dm_bytes(L1, L2);
-  const uint8_t* med = A.cm->median;
+
+  // ---- walk ----------------------------------------------------------------
+  uint32_t wa0 = 0, wa1 = 0, wa2 = 0, wa3 = 0, wb0 = 0, wb1 = 0, wb2 = 0, wb3 = 0;
+  long long qa = -(1ll << 60), qb = -(1ll << 60);  // top blocks of the windows
+  uint32_t mv = 0;  // move of step 64m + lane of the current chunk m
+  int n = 0;        // steps recorded
+  auto record = [&](int m) __attribute__((always_inline)) {
+    if (lane == (n & 63)) mv = (uint32_t)m;
+    n++;
+    if ((n & 63) == 0) r1[cap - 1 - (n - 64) - lane] = (uint8_t)mv;
+  };
   int i = L1 - 1, j = L2 - 1;
   int cb = -1, jlo = 0;
-  long long base = 0;
-#define PUT(buf, n, v) (buf)[cap - 1 - (n)++] = (uint8_t)(v)
-  for (;;) {
-    int x, y, mv;
-    if (i == 0 && j == 0) {
-      mv = 0;
-    } else if (i == 0) {
-      mv = 2;
-    } else if (j == 0) {
-      mv = 1;
-    } else {
-      const int bb = (i - 1) >> 6;
-      if (bb != cb) {
-        cb = bb;
-        jlo = full ? 0 : imax(1 + (bb << 6) - newk, 0);
-        base = stab[bb];
-      }
-      const int t = (i - 1) & 63;
-      const int s = j - jlo + t;
-      long long at = base + ((((long long)(s >> 4) << 6) + t) << 2);
-      // path cells are inside the band; the clamp only keeps a corrupt walk
-      // inside the pair's region
-      if (s < 0 || at + 4 > dm_cap) at = 0;
-      const uint32_t w = *reinterpret_cast<const uint32_t*>(dm + at);
-      mv = (w >> ((s & 15) << 1)) & 3;
+  long long base = 0;  // first 16-step block of the stripe
+  while (i != 0 && j != 0) {
+    const int bb = (i - 1) >> 6;
+    if (bb != cb) {
+      cb = bb;
+      jlo = full ? 0 : imax(1 + (bb << 6) - newk, 0);
+      base = stab[bb] >> 8;  // 64 lanes x 4 bytes per block
     }
-    if (mv == 0) { x = sa[i]; y = sb[j]; }
-    else if (mv == 1) { x = sa[i]; y = kGap; }
-    else { x = kGap; y = sb[j]; }
-    PUT(r1, n1, x);
-    PUT(r2, n2, y);
-    const int m = sw ? med[(y << 5) + x] : med[(x << 5) + y];
-    if (m != kGap) PUT(an, na, m);
-    PUT(mw, nw, m);
-    if (i == 0 && j == 0) break;
-    if (mv == 0) { i--; j--; }
-    else if (mv == 1) i--;
-    else j--;
+    const int t = (i - 1) & 63;
+    const int s = j - jlo + t;
+    // path cells are inside the band (a step before the stripe reads the
+    // diagonal code 0)
+    int code = 0;
+    if (s >= 0) {
+      const long long q = base + (s >> 4);
+      if (q > qa || q < qa - 3) {
+        if (q <= qb && q >= qb - 3) {
+          wa0 = wb0; wa1 = wb1; wa2 = wb2; wa3 = wb3;
+          qa = qb;
+        } else {
+          wa0 = dm_word(dm, dm_cap, false, q);
+          wa1 = dm_word(dm, dm_cap, false, q - 1);
+          wa2 = dm_word(dm, dm_cap, false, q - 2);
+          wa3 = dm_word(dm, dm_cap, false, q - 3);
+          qa = q;
+        }
+        // the next window loads while this one is walked
+        wb0 = dm_word(dm, dm_cap, false, qa - 4);
+        wb1 = dm_word(dm, dm_cap, false, qa - 5);
+        wb2 = dm_word(dm, dm_cap, false, qa - 6);
+        wb3 = dm_word(dm, dm_cap, false, qa - 7);
+        qb = qa - 4;
+      }
+      const int k = (int)(qa - q);
+      const uint32_t v = k < 2 ? (k == 0 ? wa0 : wa1) : (k == 2 ? wa2 : wa3);
+      const uint32_t w = __builtin_amdgcn_readlane(v, t);
+      code = (int)(w >> ((s & 15) << 1)) & 3;
+    }
+    record(code);
+    if (code == 0) {
+      i--;
+      j--;
+    } else if (code == 1) {
+      i--;
+    } else {
+      j--;
+    }
   }
-  PUT(an, na, kGap);
-#undef PUT
-  A.outlen[4 * pid + (sw ? 1 : 0)] = n1;
-  A.outlen[4 * pid + (sw ? 0 : 1)] = n2;
-  A.outlen[4 * pid + 2] = na;
-  A.outlen[4 * pid + 3] = nw;
+  // the reference's border moves: up along column 0, left along row 0, and
+  // the (0, 0) cell
+  while (i != 0) { record(1); i--; }
+  while (j != 0) { record(2); j--; }
+  record(0);
+  if ((n & 63) != 0 && lane < (n & 63)) r1[cap - 1 - (n & ~63) - lane] = (uint8_t)mv;
+  __threadfence_block();
+
+  // ---- expansion -----------------------------------------------------------
+  int ci = 0, cj = 0, ca = 0;  // rows, columns and ancestor bytes before the chunk
+  for (int c = 0; c < n; c += 64) {
+    const int s = c + lane;
+    const bool ok = s < n;
+    const int m = ok ? r1[cap - 1 - s] : 3;
+    const bool di = ok && m < 2, dj = ok && m != 1;
+    // moves before this one, packed: rows in bits 0-7, columns in 8-15
+    int tot;
+    const int ex = wave_excl_sum((di ? 1 : 0) | (dj ? 256 : 0), tot);
+    const int x = di ? sa[L1 - 1 - ci - (ex & 255)] : kGap;
+    const int y = dj ? sb[L2 - 1 - cj - (ex >> 8)] : kGap;
+    const int md = sw ? s_med[(y << 5) + x] : s_med[(x << 5) + y];
+    const bool ea = ok && md != kGap;
+    int atot;
+    const int aex = wave_excl_sum(ea ? 1 : 0, atot);
+    if (ok) {
+      r1[cap - 1 - s] = (uint8_t)x;
+      r2[cap - 1 - s] = (uint8_t)y;
+      mw[cap - 1 - s] = (uint8_t)md;
+    }
+    if (ea) an[cap - 1 - (ca + aex)] = (uint8_t)md;
+    ci += tot & 255;
+    cj += tot >> 8;
+    ca += atot;
+  }
+  if (lane == 0) {
+    an[cap - 1 - ca] = kGap;
+    A.outlen[4 * pid + (sw ? 1 : 0)] = n;
+    A.outlen[4 * pid + (sw ? 0 : 1)] = n;
+    A.outlen[4 * pid + 2] = ca + 1;
+    A.outlen[4 * pid + 3] = n;
+  }
 }
 
 // Align.count_gaps (seq_CAML_count, src/seq.c:644) for both sequences and

@@ -767,7 +767,7 @@ int batch_enqueue(poyhip_batch* b, const uint8_t* blob, const int64_t* off1, con
       B.res = b->d_res;
       B.out = d_out;
       B.outlen = d_outlen;
-      hipLaunchKernelGGL(k_linear_backtrace, dim3((t.n + 63) / 64), dim3(64), 0, st, B);
+      hipLaunchKernelGGL(k_linear_backtrace, dim3(t.n), dim3(64), 0, st, B);
     }
     HIPCHK(c, hipGetLastError());
   }
@@ -1041,7 +1041,7 @@ int batch_run_two_phase(poyhip_batch* b, const uint8_t* d_blob, const int64_t* d
     B.res = b->d_res;
     B.out = d_out;
     B.outlen = d_outlen;
-    hipLaunchKernelGGL(k_linear_backtrace, dim3((t.n + 63) / 64), dim3(64), 0, st, B);
+    hipLaunchKernelGGL(k_linear_backtrace, dim3(t.n), dim3(64), 0, st, B);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipEventSynchronize(ev[2]));
     float ms = 0.f;

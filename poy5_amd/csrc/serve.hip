@@ -338,7 +338,7 @@ int srv_launch(poyhip_server* s, SrvSlot& sl) {
       B.res = sl.d_res;
       B.out = sl.d_out;
       B.outlen = sl.d_outlen;
-      hipLaunchKernelGGL(k_linear_backtrace, dim3((n + 63) / 64), dim3(64), 0, st, B);
+      hipLaunchKernelGGL(k_linear_backtrace, dim3(n), dim3(64), 0, st, B);
       if (nkeep > 0)
         hipLaunchKernelGGL(k_pool_store, dim3(nkeep), dim3(64), 0, st, sl.d_out, dt.out_off,
                            dt.keep_cap, sl.d_outlen, dt.keep_pid, dt.keep_pool, s->d_pool, nkeep);
