@@ -229,6 +229,30 @@ def test_batch_affine_mixed_classes(ctx):
             assert np.array_equal(outs[k][3], ref["medianwg"]), (budget, k)
 
 
+def test_batch_affine_plain_stripe_fallback(ctx):
+    """Gap-free pairs outside the rotating schedule's limits run the plain
+    stripe loop: more than 128 stripes (8,300 rows) and more than 65,535
+    columns."""
+    say("test_batch_affine_plain_stripe_fallback")
+    ctx.set_cm(cost_matrix.of_transformations_and_gaps(1, 1, 3))
+    ocm = oracle.cm_dna(1, 1, 3)
+    rng = np.random.default_rng(0xFA11)
+    root = np.concatenate([[16], rng.choice(PLAIN, 8300)]).astype(np.uint8)
+    pairs = [(mutate(rng, root, 0.02), mutate(rng, root, 0.02)),
+             (np.concatenate([[16], rng.choice(PLAIN, 40)]).astype(np.uint8),
+              np.concatenate([[16], rng.choice(PLAIN, 65_600)]).astype(np.uint8))]
+    costs, outs = run_pairs(ctx, "align_affine", pairs)
+    for k, (a, b) in enumerate(pairs):
+        sw = len(a) > len(b)
+        si, sj = (b, a) if sw else (a, b)
+        ref = oracle.align_affine_3(si, sj, ocm, int(sw))
+        ra, rb = (ref["resj"], ref["resi"]) if sw else (ref["resi"], ref["resj"])
+        assert costs[k] == ref["cost"], (k, costs[k], ref["cost"])
+        assert np.array_equal(outs[k][0], ra) and np.array_equal(outs[k][1], rb), k
+        assert np.array_equal(outs[k][2], ref["median"]), k
+        assert np.array_equal(outs[k][3], ref["medianwg"]), k
+
+
 def test_batch_linear_config1_sample(ctx):
     say("test_batch_linear_config1_sample")
     ctx.set_cm(cost_matrix.of_transformations_and_gaps(1, 2))
