@@ -497,21 +497,24 @@ __device__ void rot_round_fast(RotShared& sh, int total, const uint8_t* __restri
     if (s == n63sw) c63 = n63c;
     int c0b[4], grb[4], cdb[4];
     int fvx[4], fvy[4], fvz[4], fvw[4];
-    // table entries of the block's columns, looked up together
+    // table entries of the block's columns, looked up together.  Column
+    // symbols are 1..15 past column 0 and 0 outside the sequence; column 0's
+    // leading gap reads past its row (at most into the padding after row
+    // 15), a value its cell never uses (its diagonal is HIGH_NUM).
 #pragma unroll
     for (int u = 0; u < 4; u++) {
       c0b[u] = cn[u];
       grb[u] = s_prep[cl[u]];
-      cdb[u] = s_cost[crow + (cl[u] & 15)];
+      cdb[u] = s_cost[crow + cl[u]];
+    }
+    if (b0 == 0) {
+#pragma unroll
+      for (int u = 0; u < 4; u++) cn[u] = ld_sym(rsym, col0(s + 4 + u));
     }
     {
       const int cbn = (s + 4 >= nsw) ? ncbc : cbc;
 #pragma unroll
-      for (int u = 0; u < 4; u++) {
-        const int st = s + 4 + u;
-        if (b0 == 0) cn[u] = ld_sym(rsym, col0(st));
-        cl[u] = ld_sym(rsym, st + cbn);
-      }
+      for (int u = 0; u < 4; u++) cl[u] = ld_sym(rsym, s + 4 + u + cbn);
     }
     int pack = 0;
     // One step.  The slow form also handles the uniform events of a block:
@@ -671,13 +674,12 @@ __device__ void rot_round_fast(RotShared& sh, int total, const uint8_t* __restri
 // At most 128 VGPRs: four wavefronts per SIMD (shared memory allows as many)
 template <bool kDm, bool kFast>
 __global__ __launch_bounds__(64, 4) void k_affine_fill(AffFillArgs A) {
-  // gap-free pairs only index rows 0..15 of the cost table
   // gap-free pairs only index rows and columns 0..15 of the cost table,
   // kept 16 wide (the wavefront's shared memory stays within a sixteenth
-  // of the CU's, for four wavefronts per SIMD)
+  // of the CU's, for four wavefronts per SIMD), plus 16 padding entries
   constexpr int kCostN = kFast ? 256 : 1024;
   constexpr int kRowShift = kFast ? 4 : 5;
-  __shared__ int s_cost[kCostN];
+  __shared__ int s_cost[kCostN + 16];
   __shared__ int s_ge[32];
   __shared__ int s_prep[32];
   __shared__ RotShared s_rot;
@@ -686,6 +688,7 @@ __global__ __launch_bounds__(64, 4) void k_affine_fill(AffFillArgs A) {
   if (!kFast && A.ngen != nullptr && __builtin_amdgcn_readfirstlane(*A.ngen) == 0) return;
   for (int k = lane; k < kCostN; k += 64)
     s_cost[k] = A.cm->cost[((k >> kRowShift) << 5) + (k & ((1 << kRowShift) - 1))];
+  if (lane < 16) s_cost[kCostN + lane] = 0;
   if (lane < 32) s_ge[lane] = A.cm->cost[(lane << 5) + kGap];
   if (lane < 32) s_prep[lane] = A.cm->prepend[lane];
   __syncthreads();
