@@ -438,20 +438,24 @@ int poyhip_align_affine_3(poyhip_ctx* c, const uint8_t* si, int li, const uint8_
   int any = 0;
   for (int k = 1; k < li; k++) any |= si[k];
   for (int k = 1; k < lj; k++) any |= sj[k];
-  if (any & kGap)
+  if (any & kGap) {
     hipLaunchKernelGGL((k_affine_fill<true, false>), dim3(1), dim3(64), 0, c->stream, A);
-  else
+    HIPCHK(c, hipGetLastError());
+    AffBtArgs B;
+    B.pt = t;
+    B.cm = c->d_cm;
+    B.dm = A.dm;
+    B.stab = A.stab;
+    B.res = dres;
+    B.out = dout;
+    B.outlen = doutlen;
+    hipLaunchKernelGGL(k_affine_backtrace, dim3(1), dim3(64), 0, c->stream, B);
+  } else {
+    // the gap-free kernel runs the backtrace after the fill
+    A.out = dout;
+    A.outlen = doutlen;
     hipLaunchKernelGGL((k_affine_fill<true, true>), dim3(1), dim3(64), 0, c->stream, A);
-  HIPCHK(c, hipGetLastError());
-  AffBtArgs B;
-  B.pt = t;
-  B.cm = c->d_cm;
-  B.dm = A.dm;
-  B.stab = A.stab;
-  B.res = dres;
-  B.out = dout;
-  B.outlen = doutlen;
-  hipLaunchKernelGGL(k_affine_backtrace, dim3(1), dim3(64), 0, c->stream, B);
+  }
   HIPCHK(c, hipGetLastError());
   const size_t back = kResBytes + outb;
   c->h_out.resize(back);

@@ -296,6 +296,10 @@ int srv_launch(poyhip_server* s, SrvSlot& sl) {
     A.cls = sl.d_cls;
     A.ngen = nullptr;
     A.counter = sl.d_counters;
+    // gap-free pairs: fill and backtrace in one kernel; the backtrace kernel
+    // takes the pairs with gap-bit symbols
+    A.out = sl.d_out;
+    A.outlen = sl.d_outlen;
     hipLaunchKernelGGL((k_affine_fill<true, true>), dim3(grid), dim3(64), 0, st, A);
     A.counter = sl.d_counters + 1;
     hipLaunchKernelGGL((k_affine_fill<true, false>), dim3(grid), dim3(64), 0, st, A);
@@ -307,6 +311,7 @@ int srv_launch(poyhip_server* s, SrvSlot& sl) {
     B.res = sl.d_res;
     B.out = sl.d_out;
     B.outlen = sl.d_outlen;
+    B.cls = sl.d_cls;
     hipLaunchKernelGGL(k_affine_backtrace, dim3(n), dim3(64), 0, st, B);
   } else if (op == POYHIP_OP_COST_AFFINE) {
     AffCostArgs A;
