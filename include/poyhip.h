@@ -286,18 +286,6 @@ int poyhip_sync(poyhip_ctx *ctx);
  * copies on it). */
 void *poyhip_stream(poyhip_ctx *ctx);
 
-/* ---- workloads (synthetic inputs of the benchmark configurations) ------- */
-/* SplitMix64-seeded pair generators; see the package documentation for the
- * mutation process.  Two-pass: call with blob == NULL to get the total
- * bytes, then again to fill.  Sequences are written back to back; off/len
- * arrays have 2n entries (first, second sequence of each pair). */
-int poyhip_gen_pairs(int config, int n, uint64_t seed, uint8_t *blob, int64_t *off,
-                     int *len, int64_t *total_bytes);
-/* Triples of configuration 5 (three mutations of one root); off/len arrays
- * have 3n entries. */
-int poyhip_gen_triples(int config, int n, uint64_t seed, uint8_t *blob, int64_t *off,
-                       int *len, int64_t *total_bytes);
-
 #ifdef __cplusplus
 }
 #endif

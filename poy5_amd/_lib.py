@@ -58,7 +58,8 @@ class PoyhipError(RuntimeError):
 _lock = threading.Lock()
 _lib = None
 
-# name -> (restype, argtypes); every symbol here is declared in poyhip.h
+# name -> (restype, argtypes); every symbol here is declared in poyhip.h or
+# poyhip_tools.h (benchmark generators and test hooks)
 _u8p = C.POINTER(C.c_uint8)
 _ip = C.POINTER(C.c_int)
 _i64p = C.POINTER(C.c_int64)
@@ -106,6 +107,7 @@ SIGNATURES = {
     "poyhip_server_wait": (C.c_int, [_vp, C.c_int64, C.c_int]),
     "poyhip_server_result": (C.c_int, [_vp, C.c_int64, _ip, _u8p, _u8p, C.c_int, _ip,
                                        C.POINTER(C.c_double), C.c_int]),
+    "poyhip_server_fail_next_launch": (C.c_int, [_vp]),
     "poyhip_server_stats": (C.c_int, [_vp, _i64p, _i64p, C.POINTER(C.c_double), _ip, _ip]),
     "poyhip_server_replay": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_double), _ip, _i64p, _i64p,
                                        _ip, _ip, _ip, C.POINTER(C.c_double), _i64p,

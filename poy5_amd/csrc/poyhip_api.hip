@@ -13,7 +13,7 @@
 #include <string>
 #include <vector>
 
-#include "poyhip.h"
+#include "poyhip_tools.h"
 #include "poyhip_device.h"
 #include "poyhip_layout.h"
 

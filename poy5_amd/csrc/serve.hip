@@ -125,6 +125,9 @@ struct poyhip_server {
   int64_t next_id = 1;
   bool stop = false;
   int error = 0;
+  // set by poyhip_server_fail_next_launch: the next admission fails as a
+  // launch error would, without touching the device
+  bool fail_next_launch = false;
   std::thread worker;
   // statistics
   int64_t batches = 0, pairs_done = 0;
@@ -161,6 +164,19 @@ SrvTab srv_tab(void* base, int cap) {
   return t;
 }
 constexpr size_t kSrvTabPer = 6 * 8 + 8 * 4;
+
+// Marks the requests of a slot as failed with `code`; a median a failed
+// request was to keep is marked as never coming (len -2), so submissions that
+// wait for it return instead of blocking.  Called with the lock held.
+void srv_fail(poyhip_server* s, SrvSlot& sl, int code) {
+  s->error = code;
+  for (auto* r : sl.reqs) {
+    r->status = code;
+    if (r->median_id >= 0) s->entries[r->median_id].len = -2;
+  }
+  sl.reqs.clear();
+  sl.busy = false;
+}
 
 // Finishes the requests of a completed slot.  Called with the lock held.
 void srv_finish(poyhip_server* s, SrvSlot& sl) {
@@ -382,10 +398,7 @@ void srv_loop(poyhip_server* s) {
         srv_finish(s, sl);
         progress = true;
       } else if (e != hipErrorNotReady) {
-        s->error = POYHIP_EDEVICE;
-        for (auto* r : sl.reqs) r->status = POYHIP_EDEVICE;
-        sl.reqs.clear();
-        sl.busy = false;
+        srv_fail(s, sl, POYHIP_EDEVICE);
         progress = true;
       } else {
         inflight++;
@@ -411,10 +424,10 @@ void srv_loop(poyhip_server* s) {
         s->pending.pop_front();
       }
       s->max_batch_seen = std::max(s->max_batch_seen, (int)sl.reqs.size());
-      if (srv_launch(s, sl) != POYHIP_OK) {
-        s->error = POYHIP_EDEVICE;
-        for (auto* r : sl.reqs) r->status = POYHIP_EDEVICE;
-        sl.reqs.clear();
+      const bool inject = s->fail_next_launch;
+      s->fail_next_launch = false;
+      if (inject || srv_launch(s, sl) != POYHIP_OK) {
+        srv_fail(s, sl, POYHIP_EDEVICE);
         s->cv_done.notify_all();
       } else {
         inflight++;
@@ -535,6 +548,7 @@ int poyhip_server_destroy(poyhip_server* s) {
     s->stop = true;
   }
   s->cv_work.notify_all();
+  s->cv_done.notify_all();
   if (s->worker.joinable()) s->worker.join();
   srv_free(s);
   delete s;
@@ -584,8 +598,16 @@ int poyhip_server_submit(poyhip_server* s, int op, int64_t a, int64_t b, int del
   const int64_t ne = (int64_t)s->entries.size();
   if (a < 0 || b < 0 || a >= ne || b >= ne) return set_err(s->ctx, POYHIP_EINVAL, "unknown sequence id");
   if (s->entries[a].len < 0 || s->entries[b].len < 0) {
-    // a median that is still being computed: wait until it exists
-    s->cv_done.wait(lk, [&] { return s->entries[a].len >= 0 && s->entries[b].len >= 0; });
+    // a median that is still being computed: wait until it exists, its
+    // producer failed (len -2) or the server stops
+    s->cv_done.wait(lk, [&] {
+      return s->error || s->stop || s->entries[a].len == -2 || s->entries[b].len == -2 ||
+             (s->entries[a].len >= 0 && s->entries[b].len >= 0);
+    });
+    if (s->error) return set_err(s->ctx, s->error, "the server failed while this request waited");
+    if (s->stop) return set_err(s->ctx, POYHIP_ESTATE, "the server is stopping");
+    if (s->entries[a].len < 0 || s->entries[b].len < 0)
+      return set_err(s->ctx, POYHIP_EDEVICE, "the request computing a named median failed");
   }
   SrvRequest* r = new SrvRequest();
   r->id = s->next_id++;
@@ -605,7 +627,7 @@ int poyhip_server_submit(poyhip_server* s, int op, int64_t a, int64_t b, int del
   }
   if (r->dm_bytes > s->slot_dm || srv_out_bytes(op, r->la, r->lb) > s->slot_out) {
     delete r;
-    return set_err(s->ctx, POYHIP_ECAP, "request exceeds a slot's memory budget");
+    return set_err(s->ctx, POYHIP_EBUSY, "request exceeds a slot's memory budget");
   }
   if (flags & POYHIP_REQ_KEEP_MEDIAN) {
     const int cap = (int)lin_out_cap(r->la, r->lb);
@@ -639,10 +661,11 @@ int poyhip_server_wait(poyhip_server* s, int64_t req, int timeout_ms) {
   auto it = s->reqs.find(req);
   if (it == s->reqs.end()) return POYHIP_EINVAL;
   SrvRequest* r = it->second;
-  auto done = [&] { return r->status == 2 || r->status < 0; };
+  auto done = [&] { return r->status == 2 || r->status < 0 || s->stop; };
   if (timeout_ms < 0) s->cv_done.wait(lk, done);
   else if (!s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), done)) return POYHIP_EBUSY;
-  return r->status < 0 ? r->status : POYHIP_OK;
+  if (r->status < 0) return r->status;
+  return r->status == 2 ? POYHIP_OK : set_err(s->ctx, POYHIP_ESTATE, "the server is stopping");
 }
 
 int poyhip_server_result(poyhip_server* s, int64_t req, int* cost, uint8_t* r1, uint8_t* r2,
@@ -667,6 +690,13 @@ int poyhip_server_result(poyhip_server* s, int64_t req, int* cost, uint8_t* r1, 
     s->reqs.erase(it);
     delete r;
   }
+  return POYHIP_OK;
+}
+
+int poyhip_server_fail_next_launch(poyhip_server* s) {
+  if (!s) return POYHIP_EINVAL;
+  std::lock_guard<std::mutex> g(s->m);
+  s->fail_next_launch = true;
   return POYHIP_OK;
 }
 

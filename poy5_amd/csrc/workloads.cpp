@@ -17,7 +17,7 @@
 #include <thread>
 #include <vector>
 
-#include "poyhip.h"
+#include "poyhip_tools.h"
 
 namespace {
 

@@ -60,10 +60,13 @@ def test_deltaw_calc_product_matches_oracle():
         assert got == want, (l1, l2, dw)
 
 
-def _header_functions():
-    text = open(os.path.join(ROOT, "include", "poyhip.h")).read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return set(re.findall(r"\b(poyhip_\w+)\s*\(", text))
+def _header_functions(*names):
+    found = set()
+    for name in names or ("poyhip.h", "poyhip_tools.h"):
+        text = open(os.path.join(ROOT, "include", name)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        found |= set(re.findall(r"\b(poyhip_\w+)\s*\(", text))
+    return found
 
 
 def test_abi_exports_every_declared_symbol():
@@ -72,8 +75,10 @@ def test_abi_exports_every_declared_symbol():
     L = C.CDLL(_lib.LIB_PATH)
     for name in declared:
         assert hasattr(L, name), name
-    # the ctypes binding covers exactly what the header declares
+    # the ctypes binding covers exactly what the headers declare
     assert declared == set(_lib.SIGNATURES), declared ^ set(_lib.SIGNATURES)
+    # benchmark generators and test hooks stay out of the drop-in header
+    assert not any(n.startswith("poyhip_gen_") for n in _header_functions("poyhip.h"))
 
 
 def test_library_loads_without_gpu():

@@ -66,14 +66,22 @@ def pair_cases(seed, count, maxlen, gappy=0.0):
     return cases
 
 
+# (substitution, gap extension, gap opening): the minimum non-zero cost
+# (the T-doubling unit delta) is 1 for the first and third model and 2 for
+# the second, whose extension also differs from its substitution
+AFFINE_MODELS = [(1, 1, 3), (2, 2, 4), (1, 3, 2)]
+
+
+@pytest.mark.parametrize("model", AFFINE_MODELS)
 @pytest.mark.parametrize("gappy", [0.0, 0.15])
-def test_align_affine_3_dropin(ctx, gappy):
-    say(f"test_align_affine_3_dropin gappy={gappy}")
-    cm = cost_matrix.of_transformations_and_gaps(1, 1, 3)
+def test_align_affine_3_dropin(ctx, gappy, model):
+    say(f"test_align_affine_3_dropin gappy={gappy} model={model}")
+    cm = cost_matrix.of_transformations_and_gaps(*model)
     ctx.set_cm(cm)
-    ocm = oracle.cm_dna(1, 1, 3)
+    ocm = oracle.cm_dna(*model)
     al = Align(ctx)
-    for k, (a, b) in enumerate(pair_cases(11 + int(gappy * 100), 60, 400, gappy)):
+    n = 60 if model == (1, 1, 3) else 30
+    for k, (a, b) in enumerate(pair_cases(11 + int(gappy * 100), n, 400, gappy)):
         if len(a) > len(b):
             a, b = b, a
         for sw in (0, 1):
@@ -86,10 +94,11 @@ def test_align_affine_3_dropin(ctx, gappy):
             assert np.array_equal(mwg, ref["medianwg"]), (k, sw)
 
 
-def test_cost_affine_3_dropin(ctx):
-    say("test_cost_affine_3_dropin")
-    ctx.set_cm(cost_matrix.of_transformations_and_gaps(1, 1, 3))
-    ocm = oracle.cm_dna(1, 1, 3)
+@pytest.mark.parametrize("model", AFFINE_MODELS)
+def test_cost_affine_3_dropin(ctx, model):
+    say(f"test_cost_affine_3_dropin model={model}")
+    ctx.set_cm(cost_matrix.of_transformations_and_gaps(*model))
+    ocm = oracle.cm_dna(*model)
     al = Align(ctx)
     for k, (a, b) in enumerate(pair_cases(5, 40, 300, 0.1)):
         assert al.cost_2_affine(a, b) == oracle.cost_affine_3(a, b, ocm), k
@@ -129,10 +138,11 @@ def test_medians_dropin(ctx):
             assert al.max_cost_2(o1, o2) == want, k
 
 
-def test_batch_affine_config2_sample(ctx):
-    say("test_batch_affine_config2_sample")
-    ctx.set_cm(cost_matrix.of_transformations_and_gaps(1, 1, 3))
-    ocm = oracle.cm_dna(1, 1, 3)
+@pytest.mark.parametrize("model", AFFINE_MODELS)
+def test_batch_affine_config2_sample(ctx, model):
+    say(f"test_batch_affine_config2_sample model={model}")
+    ctx.set_cm(cost_matrix.of_transformations_and_gaps(*model))
+    ocm = oracle.cm_dna(*model)
     pairs = workloads.pairs_list(2, 96)
     pairs = [p for p in pairs if len(p[0]) * len(p[1]) < 1_200_000][:48]
     costs, outs = run_pairs(ctx, "align_affine", pairs)
@@ -271,19 +281,22 @@ def test_batch_linear_config1_sample(ctx):
 
 def test_batch_linear_config4_long(ctx):
     """Long banded pairs of the config-4 process (16k+ symbols, 10% IUPAC,
-    sub 1 / indel 1): the two shortest of 12 draws, bit-exact."""
+    sub 1 / indel 1): the six shortest of 24 draws, bit-exact in cost,
+    aligned pair, ancestor median and median with gaps."""
     say("test_batch_linear_config4_long")
     ctx.set_cm(cost_matrix.of_transformations_and_gaps(1, 1))
     ocm = oracle.cm_dna(1, 1)
-    pairs = workloads.pairs_list(4, 12)
+    pairs = workloads.pairs_list(4, 24)
     pairs.sort(key=lambda p: len(p[0]) * len(p[1]))
-    pairs = pairs[:2]
+    pairs = pairs[:6]
     costs, outs = run_pairs(ctx, "align_2d", pairs)
     for k, (a, b) in enumerate(pairs):
         o1, o2, oc, st = oracle.align_2(a, b, ocm)
         assert not st.full_plane
         assert costs[k] == oc, (k, costs[k], oc)
         assert np.array_equal(outs[k][0], o1) and np.array_equal(outs[k][1], o2), k
+        assert np.array_equal(outs[k][2], oracle.ancestor_2(o1, o2, ocm)), k
+        assert np.array_equal(outs[k][3], oracle.median_2_with_gaps(o1, o2, ocm)), k
 
 
 def test_batch_linear_two_phase(ctx):

@@ -115,3 +115,116 @@ def test_affine_requests_match_oracle(ctx):
     finally:
         srv.close()
         c2.close()
+
+
+def test_admission_refuses_over_budget_and_serves_the_rest(ctx):
+    """A request whose direction store exceeds a slot's budget is refused
+    with POYHIP_EBUSY at submission; the requests around it still match the
+    oracle."""
+    print("test_admission_refuses_over_budget_and_serves_the_rest", flush=True)
+    from poy5_amd import _lib
+    ocm = oracle.cm_dna(1, 2)
+    rng = np.random.default_rng(21)
+    small = [np.concatenate([[16], rng.choice(spr.BASES, n)]).astype(np.uint8)
+             for n in (150, 170, 190, 210)]
+    big = np.concatenate([[16], rng.choice(spr.BASES, 3000)]).astype(np.uint8)
+    # ~0.25 B per cell: 3001 x 3001 cells need ~2.3 MB, over the 1 MiB budget
+    srv = Server(ctx, n_slots=2, max_batch_pairs=64, max_len=4096, slot_dm_bytes=1 << 20,
+                 pool_bytes=4 << 20)
+    try:
+        ids = [srv.put(s) for s in small]
+        bid = srv.put(big)
+        ok = []
+        for k in range(len(ids) - 1):
+            ok.append((k, srv.submit("align_2d", ids[k], ids[k + 1])[0]))
+            if k == 1:
+                with pytest.raises(_lib.PoyhipError) as e:
+                    srv.submit("align_2d", bid, bid)
+                assert e.value.code == -6  # POYHIP_EBUSY
+        for k, r in ok:
+            srv.wait(r)
+            c, r1, r2, _ = srv.result(r)
+            o1, o2, oc, _ = oracle.align_2(small[k], small[k + 1], ocm)
+            assert c == oc and np.array_equal(r1, o1) and np.array_equal(r2, o2), k
+    finally:
+        srv.close()
+
+
+def test_failed_producer_releases_waiting_submissions(ctx):
+    """A request that keeps its median fails at launch (forced from the host,
+    no kernel runs); a later submission naming that median returns an error
+    instead of blocking, and so does every submission after it."""
+    print("test_failed_producer_releases_waiting_submissions", flush=True)
+    import threading
+    from poy5_amd import _lib
+    rng = np.random.default_rng(22)
+    seqs = [np.concatenate([[16], rng.choice(spr.BASES, 120)]).astype(np.uint8)
+            for _ in range(3)]
+    srv = Server(ctx, n_slots=1, max_batch_pairs=16, max_len=1024, slot_dm_bytes=8 << 20,
+                 pool_bytes=1 << 20)
+    try:
+        ids = [srv.put(s) for s in seqs]
+        ctx.chk(srv.L.poyhip_server_fail_next_launch(srv.h))
+        r, m = srv.submit("align_2d", ids[0], ids[1], keep_median=True)
+        out = {}
+
+        def dependent():
+            try:
+                srv.submit("cost_2d", m, ids[2])
+                out["rc"] = 0
+            except _lib.PoyhipError as e:
+                out["rc"] = e.code
+
+        t = threading.Thread(target=dependent, daemon=True)
+        t.start()
+        t.join(timeout=30)
+        assert not t.is_alive(), "submission naming a failed median blocked"
+        assert out["rc"] == -3  # POYHIP_EDEVICE
+        with pytest.raises(_lib.PoyhipError) as e:
+            srv.wait(r)
+        assert e.value.code == -3
+    finally:
+        srv.close()
+
+
+@pytest.mark.parametrize("model", [(1, 1, 3), (2, 1, 4), (1, 3, 2)])
+def test_affine_gap_bit_and_gap_free_mix_match_oracle(model):
+    """Serving affine requests: pairs with gap-bit symbols (codes 17-31) and
+    gap-free pairs in the same slot batch; cost, aligned pair and, for align
+    requests, the kept alignment match oracle.align_affine_3 under several
+    cost models (minimum non-zero cost 1 and 2)."""
+    print(f"test_affine_gap_bit_and_gap_free_mix_match_oracle {model}", flush=True)
+    c2 = Context(0)
+    c2.set_cm(cost_matrix.of_transformations_and_gaps(*model))
+    ocm = oracle.cm_dna(*model)
+    rng = np.random.default_rng(23 + model[0] + 10 * model[2])
+    codes = np.array([1, 2, 4, 8, 1, 2, 4, 8, 3, 5, 15], np.uint8)
+    gapc = np.arange(17, 32, dtype=np.uint8)
+    seqs = []
+    for k in range(16):
+        s = rng.choice(codes, int(rng.integers(80, 400)))
+        if k % 4 >= 2:  # pairs (0,1), (4,5), ... stay gap-free
+            m = rng.random(len(s)) < 0.15
+            s[m] = rng.choice(gapc, int(m.sum()))
+        seqs.append(np.concatenate([[16], s]).astype(np.uint8))
+    srv = Server(c2, n_slots=1, max_batch_pairs=64, max_len=1024, slot_dm_bytes=64 << 20,
+                 pool_bytes=2 << 20)
+    try:
+        ids = [srv.put(s) for s in seqs]
+        reqs = [(k, srv.submit("align_affine", ids[k], ids[k + 1])[0],
+                 srv.submit("cost_affine", ids[k], ids[k + 1])[0]) for k in range(len(ids) - 1)]
+        for k, ra, rc in reqs:
+            a, b = seqs[k], seqs[k + 1]
+            srv.wait(ra)
+            c, r1, r2, _ = srv.result(ra)
+            sw = len(a) > len(b)
+            si, sj = (b, a) if sw else (a, b)
+            ref = oracle.align_affine_3(si, sj, ocm, int(sw))
+            ra_, rb_ = (ref["resj"], ref["resi"]) if sw else (ref["resi"], ref["resj"])
+            assert c == ref["cost"], k
+            assert np.array_equal(r1, ra_) and np.array_equal(r2, rb_), k
+            srv.wait(rc)
+            assert srv.result(rc)[0] == oracle.cost_affine_3(a, b, ocm), k
+    finally:
+        srv.close()
+        c2.close()
