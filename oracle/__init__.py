@@ -75,6 +75,9 @@ def lib():
             getattr(L, f).argtypes = [u8p, u8p, C.c_int, C.POINTER(OrcCM)]
         L.orc_align_affine_3.argtypes = [u8p, C.c_int, u8p, C.c_int, C.POINTER(OrcCM), C.c_int,
                                          u8p, ip, u8p, ip, u8p, u8p, ip, C.POINTER(OrcStats)]
+        L.orc_align_affine_round.argtypes = [u8p, C.c_int, u8p, C.c_int, C.POINTER(OrcCM),
+                                             C.c_int, C.c_int, u8p, ip, u8p, ip, u8p, u8p, ip,
+                                             C.POINTER(OrcStats)]
         L.orc_cost_affine_3.argtypes = [u8p, C.c_int, u8p, C.c_int, C.POINTER(OrcCM)]
         L.orc_count_gaps.argtypes = [u8p, C.c_int]
         L.orc_deltawh.argtypes = [u8p, C.c_int, u8p, C.c_int, C.c_int, C.c_int]
@@ -188,6 +191,23 @@ def align_affine_3(si, sj, cm, swaped):
                                  P(bufs[2]), P(bufs[3]), C.byref(al), C.byref(st))
     if c == -1 and len(b) < len(a):
         raise ValueError("pass the shorter one as first")
+    return dict(cost=c, median=bufs[0][:ml.value].copy(), medianwg=bufs[1][:mwl.value].copy(),
+                resi=bufs[2][:al.value].copy(), resj=bufs[3][:al.value].copy(), stats=st)
+
+
+def align_affine_round(si, sj, cm, swaped, p):
+    """One band round at half-width parameter p on fresh rows (the first
+    round of align_affine_3's doubling) plus backtrace_aff."""
+    a, ap = _u8(si)
+    b, bp = _u8(sj)
+    cap = len(a) + len(b) + 2
+    bufs = [np.zeros(cap, np.uint8) for _ in range(4)]
+    ml, mwl, al = C.c_int(), C.c_int(), C.c_int()
+    st = OrcStats()
+    P = lambda x: x.ctypes.data_as(C.POINTER(C.c_uint8))  # noqa: E731
+    c = lib().orc_align_affine_round(ap, len(a), bp, len(b), C.byref(cm), int(swaped), int(p),
+                                     P(bufs[0]), C.byref(ml), P(bufs[1]), C.byref(mwl),
+                                     P(bufs[2]), P(bufs[3]), C.byref(al), C.byref(st))
     return dict(cost=c, median=bufs[0][:ml.value].copy(), medianwg=bufs[1][:mwl.value].copy(),
                 resi=bufs[2][:al.value].copy(), resj=bufs[3][:al.value].copy(), stats=st)
 

@@ -902,10 +902,32 @@ static void backtrace_aff(const uint16_t *dmbase, const uint8_t *si, int Li,
     if (median->len == 0 || GAP != rs_first(median)) rs_prepend(median, GAP);
 }
 
+static int align_affine_impl(const uint8_t *si, int Li, const uint8_t *sj, int Lj,
+                             const orc_cm *cm, int swaped, uint8_t *median, int *mlen,
+                             uint8_t *medianwg, int *mwglen, uint8_t *resi, uint8_t *resj,
+                             int *alen, orc_stats *st, int fixed_p);
+
 int orc_align_affine_3(const uint8_t *si, int Li, const uint8_t *sj, int Lj,
                        const orc_cm *cm, int swaped, uint8_t *median, int *mlen,
                        uint8_t *medianwg, int *mwglen, uint8_t *resi, uint8_t *resj,
                        int *alen, orc_stats *st) {
+    return align_affine_impl(si, Li, sj, Lj, cm, swaped, median, mlen, medianwg, mwglen,
+                             resi, resj, alen, st, -1);
+}
+
+int orc_align_affine_round(const uint8_t *si, int Li, const uint8_t *sj, int Lj,
+                           const orc_cm *cm, int swaped, int p, uint8_t *median, int *mlen,
+                           uint8_t *medianwg, int *mwglen, uint8_t *resi, uint8_t *resj,
+                           int *alen, orc_stats *st) {
+    if (p < 0) return -1;
+    return align_affine_impl(si, Li, sj, Lj, cm, swaped, median, mlen, medianwg, mwglen,
+                             resi, resj, alen, st, p);
+}
+
+static int align_affine_impl(const uint8_t *si, int Li, const uint8_t *sj, int Lj,
+                             const orc_cm *cm, int swaped, uint8_t *median, int *mlen,
+                             uint8_t *medianwg, int *mwglen, uint8_t *resi, uint8_t *resj,
+                             int *alen, orc_stats *st, int fixed_p) {
     orc_stats local;
     if (!st) st = &local;
     memset(st, 0, sizeof *st);
@@ -958,7 +980,7 @@ int orc_align_affine_3(const uint8_t *si, int Li, const uint8_t *sj, int Lj,
     int T = (x.lenj - x.leni + 1) * delta;
     int res = 0;
     for (;;) {
-        int p = (T - (x.lenj - x.leni)) / 2;
+        int p = fixed_p >= 0 ? fixed_p : (T - (x.lenj - x.leni)) / 2;
         int gap_num = aff_test(&x, p, &res);
         st->rounds++;
         st->final_T = T;
@@ -967,7 +989,7 @@ int orc_align_affine_3(const uint8_t *si, int Li, const uint8_t *sj, int Lj,
         if (gap_num - (x.lenj - x.leni) < st->min_gap_excess)
             st->min_gap_excess = gap_num - (x.lenj - x.leni);
         int newp = (2 * T - (x.lenj - x.leni)) / 2;
-        if (gap_num < p || newp - x.lenj + 1 >= 0) break;
+        if (fixed_p >= 0 || gap_num < p || newp - x.lenj + 1 >= 0) break;
         T *= 2;
     }
     st->cells = x.cells;

@@ -132,6 +132,14 @@ int orc_align_affine_3(const uint8_t *si, int Li, const uint8_t *sj, int Lj,
                        const orc_cm *cm, int swaped,
                        uint8_t *median, int *mlen, uint8_t *medianwg, int *mwglen,
                        uint8_t *resi, uint8_t *resj, int *alen, orc_stats *st);
+/* One band round of the above at a given p (band half-width min(p, Li-2)),
+ * as the first round algn_newkk_increaseT_aff (2311) runs it, then
+ * backtrace_aff: the fill of algn_newkk_test_aff (2186) on freshly
+ * initialised rows.  Test infrastructure for band-restricted pins. */
+int orc_align_affine_round(const uint8_t *si, int Li, const uint8_t *sj, int Lj,
+                           const orc_cm *cm, int swaped, int p,
+                           uint8_t *median, int *mlen, uint8_t *medianwg, int *mwglen,
+                           uint8_t *resi, uint8_t *resj, int *alen, orc_stats *st);
 /* algn_CAML_cost_affine_3 (algn.c:2457): full affine DP without backtrace,
  * either order. */
 int orc_cost_affine_3(const uint8_t *si, int Li, const uint8_t *sj, int Lj,

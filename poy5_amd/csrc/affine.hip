@@ -1263,7 +1263,15 @@ __global__ __launch_bounds__(64) void k_affine_cost_full(AffCostArgs A) {
             const int shx = (j == 1) ? gr : (((jprev & kGap) && !(jc & kGap)) ? sj_go + gr : gr);
             int ext = lH + shx, opn = lCB + sj_go + gr;
             H = ext < opn ? ext : opn;
-            ext = uV + si_vext; opn = uCB + si_go + si_ge;
+            // algn_CAML_cost_affine_3 carves its state arrays 2*largest ints
+            // apart but fills them with a row offset of largest + 1
+            // (src/algn.c:2488-2491 with algn_fill_plane_3_aff_nobt's
+            // offset, 2006): an even row's start-of-row writes of H[0] and
+            // E[0] overwrite the odd row's V and C at the last column, so
+            // V at (even i, lenj) has only HIGH sources
+            const bool lost = (j == lenj) && ((i & 1) == 0);
+            ext = (lost ? kHigh : uV) + si_vext;
+            opn = (lost ? kHigh : uCB) + si_go + si_ge;
             V = ext < opn ? ext : opn;
             const bool flag = (kGap & ic) && (kGap & jc);
             const bool flag2 = !(kGap & ip) && !(kGap & jprev);
@@ -1273,9 +1281,13 @@ __global__ __launch_bounds__(64) void k_affine_cost_full(AffCostArgs A) {
             E = ext < opn ? ext : opn;
             const int d = s_cost[(si_ng << 5) + (jc & 15)];
             const int extra = (sj_go < si_go) ? si_go : sj_go;
+            // with lenj = 1 the same overlap makes an odd row's own V and
+            // E of column 1 land on the H and V of column 0 of the row
+            // above before the aligned state reads them
+            const bool self1 = (lenj == 1) && ((i & 1) == 1);
             int cb = dCB + d;
-            const int fv = dV + d + ((ic == si_ng) ? 0 : sj_go);
-            const int fh = dH + d + ((jc == (jc & 15)) ? 0 : si_go);
+            const int fv = (self1 ? E : dV) + d + ((ic == si_ng) ? 0 : sj_go);
+            const int fh = (self1 ? V : dH) + d + ((jc == (jc & 15)) ? 0 : si_go);
             const int fd = dED + d + extra;
             if (cb > fv) cb = fv;
             if (cb > fh) cb = fh;

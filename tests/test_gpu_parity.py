@@ -104,6 +104,35 @@ def test_cost_affine_3_dropin(ctx, model):
         assert al.cost_2_affine(a, b) == oracle.cost_affine_3(a, b, ocm), k
 
 
+def test_cost_affine_3_row_overlap(ctx):
+    """algn_CAML_cost_affine_3's overlapping row buffers (the last column's V
+    of even rows has only HIGH sources; with one column the aligned state of
+    odd rows reads the cell's own V and E): known pairs where this changes
+    the cost, then thousands of tiny pairs with gap-bit symbols, through
+    the batch and the drop-in, against the oracle."""
+    say("test_cost_affine_3_row_overlap")
+    rng = np.random.default_rng(0x0E7A)
+    for model in AFFINE_MODELS:
+        ctx.set_cm(cost_matrix.of_transformations_and_gaps(*model))
+        ocm = oracle.cm_dna(*model)
+        known = [([16, 4, 1, 8, 18], [16, 4, 24, 20, 17, 2]),
+                 ([16, 4, 24, 1, 24], [16, 2, 20, 17, 17, 24, 1]),
+                 ([16, 3, 18], [16, 24, 17, 20, 24, 4])]
+        pairs = [(np.array(a, np.uint8), np.array(b, np.uint8)) for a, b in known]
+        for _ in range(3000):
+            a = rand_seq(rng, int(rng.integers(0, 7)), gappy=0.3, iupac=0.1)
+            b = rand_seq(rng, int(rng.integers(0, 7)), gappy=0.3, iupac=0.1)
+            pairs.append((a, b))
+        costs, _ = run_pairs(ctx, "cost_affine", pairs)
+        want = np.array([oracle.cost_affine_3(a, b, ocm) for a, b in pairs])
+        bad = np.nonzero(costs != want)[0]
+        assert len(bad) == 0, (model, [(list(pairs[k][0]), list(pairs[k][1])) for k in bad[:5]])
+        al = Align(ctx)
+        for k in range(0, len(pairs), 60):
+            a, b = pairs[k]
+            assert al.cost_2_affine(a, b) == want[k], (model, k)
+
+
 @pytest.mark.parametrize("tcm", [(1, 2), (1, 1)])
 def test_align_2_linear(ctx, tcm):
     say(f"test_align_2_linear tcm={tcm}")

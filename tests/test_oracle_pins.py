@@ -265,3 +265,162 @@ def test_3d_first_plane_vs_brute():
         assert [x for x in r2 if x != 16] == [x for x in ints(s2) if x != 16]
         assert [x for x in r3 if x != 16] == [x for x in ints(s3) if x != 16]
 
+
+
+# ---- four-state affine model by path enumeration (tests/brute_affine.py) ----
+import brute_affine  # noqa: E402
+
+AFF_MODELS = [(1, 1, 3), (2, 2, 4), (1, 3, 2)]
+
+
+def _bmodel(a, b, cm, mode, newk=None):
+    return brute_affine.Model(a, b, list(cm.cost), list(cm.prepend), list(cm.median),
+                              cm.gap_open, mode, newk)
+
+
+@pytest.mark.parametrize("model", AFF_MODELS)
+@pytest.mark.parametrize("gapbit", [0.0, 0.3])
+def test_affine_band_round_tie_exact_vs_paths(model, gapbit):
+    """One band round at every half-width, both swap flags: cost and the
+    exact aligned pair, median and median with gaps equal the walk of
+    backtrace_aff's mode machine over flags derived from path-enumerated
+    state values (todo priority V>H>block>align, swapped H>V; runs end on
+    END_* or HORIZONTAL_EQ_VERTICAL; ALIGN_TO_* priority likewise).  With
+    gap-bit symbols this covers POY's gap-opening rules and the gap-gap block
+    state, which the textbook model does not have."""
+    cm = oracle.cm_dna(*model)
+    rng = np.random.default_rng(300 + model[0] + 10 * model[2] + int(gapbit * 10))
+    n = 0
+    for _ in range(60):
+        la = int(rng.integers(0, 6))
+        lb = int(rng.integers(la, 7))
+        a, b = rseq(rng, la, 0.1, gapbit), rseq(rng, lb, 0.1, gapbit)
+        for p in range(0, 6):
+            newk = p if p < len(a) - 1 else len(a) - 2
+            bm = _bmodel(a, b, cm, "band", newk)
+            for sw in (0, 1):
+                r = oracle.align_affine_round(a, b, cm, sw, p)
+                med, mwg, ri, rj = bm.walk(sw)
+                assert r["cost"] == bm.cost_at_end(), (ints(a), ints(b), p, sw)
+                assert ints(r["resi"]) == ri and ints(r["resj"]) == rj, (ints(a), ints(b), p, sw)
+                assert ints(r["median"]) == med and ints(r["medianwg"]) == mwg
+                n += 1
+    assert n > 600
+
+
+@pytest.mark.parametrize("model", AFF_MODELS)
+def test_affine_doubling_result_tie_exact_vs_paths(model):
+    """The full Ukkonen doubling (algn_CAML_align_affine_3) on inputs without
+    gap-bit symbols: its result is the path-enumerated model's walk at the
+    final round's band (rows carried over from earlier rounds only touch the
+    gap-gap block state and row 0's H, which cannot win without gap bits)."""
+    cm = oracle.cm_dna(*model)
+    rng = np.random.default_rng(400 + model[0] + 10 * model[2])
+    for _ in range(120):
+        la = int(rng.integers(0, 6))
+        lb = int(rng.integers(la, 7))
+        a, b = rseq(rng, la, 0.15), rseq(rng, lb, 0.15)
+        for sw in (0, 1):
+            r = oracle.align_affine_3(a, b, cm, sw)
+            bm = _bmodel(a, b, cm, "band", r["stats"].final_k)
+            med, mwg, ri, rj = bm.walk(sw)
+            assert r["cost"] == bm.cost_at_end(), (ints(a), ints(b), sw)
+            assert ints(r["resi"]) == ri and ints(r["resj"]) == rj, (ints(a), ints(b), sw)
+            assert ints(r["median"]) == med and ints(r["medianwg"]) == mwg
+
+
+@pytest.mark.parametrize("model", AFF_MODELS)
+@pytest.mark.parametrize("gapbit", [0.0, 0.3])
+def test_affine_cost_only_vs_paths(model, gapbit):
+    """algn_CAML_cost_affine_3 (full DP, no directions): the minimum over
+    every four-state path under POY's move costs, including the effect of
+    the stub's overlapping row buffers; either argument order."""
+    cm = oracle.cm_dna(*model)
+    rng = np.random.default_rng(500 + model[0] + 10 * model[2] + int(gapbit * 10))
+    overlap_mattered = 0
+    for _ in range(300):
+        la = int(rng.integers(0, 6))
+        lb = int(rng.integers(la, 7))
+        a, b = rseq(rng, la, 0.1, gapbit), rseq(rng, lb, 0.1, gapbit)
+        # the stub takes the shorter sequence as rows, the first on a tie
+        want = _bmodel(a, b, cm, "full").cost_at_end()
+        assert oracle.cost_affine_3(a, b, cm) == want, (ints(a), ints(b))
+        rows, cols = (b, a) if len(b) == len(a) else (a, b)
+        assert oracle.cost_affine_3(b, a, cm) == _bmodel(rows, cols, cm, "full").cost_at_end()
+        clean = brute_affine.Model(a, b, list(cm.cost), list(cm.prepend), list(cm.median),
+                                   cm.gap_open, "full", overlap=False).cost_at_end()
+        assert clean <= want  # the overlap only removes candidates
+        overlap_mattered += clean < want
+    if gapbit:
+        assert overlap_mattered > 0  # the overlap is exercised, not vacuous
+
+
+def test_affine_cost_only_overlap_known_cases():
+    """Pairs where the stub's overlapping row buffers change the cost-only
+    result (found by the path enumeration): the clean four-state minimum is
+    lower than what algn_CAML_cost_affine_3 computes."""
+    cases = [((1, 1, 3), [16, 4, 1, 8, 18], [16, 4, 24, 20, 17, 2], 4, 5),
+             ((1, 3, 2), [16, 4, 24, 1, 24], [16, 2, 20, 17, 17, 24, 1], 1, 4),
+             ((1, 3, 2), [16, 3, 18], [16, 24, 17, 20, 24, 4], 3, 4)]
+    for model, a, b, clean, poy in cases:
+        cm = oracle.cm_dna(*model)
+        a, b = np.array(a, np.uint8), np.array(b, np.uint8)
+        assert oracle.cost_affine_3(a, b, cm) == poy
+        assert _bmodel(a, b, cm, "full").cost_at_end() == poy
+        assert brute_affine.Model(a, b, list(cm.cost), list(cm.prepend), list(cm.median),
+                                  cm.gap_open, "full", overlap=False).cost_at_end() == clean
+
+
+# ---- three-way cube by path enumeration (tests/brute_cube.py) -------------
+import brute_cube  # noqa: E402
+
+
+@pytest.mark.parametrize("tcm", [(1, 1), (1, 2), (2, 1)])
+def test_3d_cube_vs_path_enumeration(tcm):
+    """Every plane of algn_fill_cube, drifted row pointers included: the
+    cost, the whole direction cube (first minimising move in the
+    reference's candidate order) and backtrack_3d's walk equal the
+    path-enumerated model, for cubes up to 4 x 5 x 5 (planes >= 1 with their
+    lags of i-1 and i rows, L1 = 2 included)."""
+    c3 = oracle.cm3_from_2d(oracle.cm_dna(*tcm))
+    cost3 = list(c3.cost)
+    rng = np.random.default_rng(600 + 10 * tcm[0] + tcm[1])
+    planes = 0
+    for t in range(150):
+        n1 = int(rng.integers(0, 4)) if t % 3 else 1  # every third cube has L1 = 2
+        s1 = rseq(rng, n1, 0.2)
+        s2 = rseq(rng, int(rng.integers(0, 5)), 0.2)
+        s3 = rseq(rng, int(rng.integers(0, 5)), 0.2)
+        cube = brute_cube.Cube(s1, s2, s3, cost3)
+        cost, dm = oracle.nw_3d(s1, s2, s3, c3)
+        assert cost == cube.cost(), (ints(s1), ints(s2), ints(s3))
+        L1, L2, L3 = len(s1), len(s2), len(s3)
+        flat = np.asarray(dm).reshape(-1)
+        for i in range(L1):
+            for r in range(L2):
+                for k in range(L3):
+                    if (i, r, k) != (0, 0, 0):
+                        assert int(flat[(i * L2 + r) * L3 + k]) == cube.dm_code(i, r, k), \
+                            (ints(s1), ints(s2), ints(s3), i, r, k)
+        want = cube.walk()
+        got = oracle.backtrack_3d(s1, s2, s3, dm)
+        assert (want is None) == (got is None)
+        if want is not None:
+            assert [ints(x) for x in got] == [list(x) for x in want]
+        planes += L1 - 1
+    assert planes > 100
+
+
+def test_3d_one_row_planes_all_read_plane_zero():
+    """With L2 = 1 the drifted pointers make every plane read plane 0, so
+    the cube's cost depends on s1 only through its last symbol."""
+    c3 = oracle.cm3_from_2d(oracle.cm_dna(1, 2))
+    rng = np.random.default_rng(17)
+    s2 = np.array([16], np.uint8)
+    for _ in range(40):
+        s3 = rseq(rng, int(rng.integers(0, 8)), 0.2)
+        last = int(rng.choice(BASES))
+        ref, _ = oracle.nw_3d(np.array([16, last], np.uint8), s2, s3, c3)
+        for n in (2, 3, 6):
+            s1 = np.concatenate([rseq(rng, n - 1, 0.3), [last]]).astype(np.uint8)
+            assert oracle.nw_3d(s1, s2, s3, c3)[0] == ref
