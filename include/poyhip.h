@@ -107,6 +107,22 @@ int poyhip_align_affine_3(poyhip_ctx *ctx, const uint8_t *si, int li,
                           uint8_t *resi, uint8_t *resj, int *res_len, int cap,
                           int *cost);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): poyhip_align_affine_3 with the same arguments and
+ * results, whose staging, direction store, workspace and outputs live in a
+ * fixed 64 MiB device array per device, held until the process ends (about
+ * li * lj + 16 * lj + 6 * (li + lj) bytes per pair: pairs up to ~8,000 x
+ * 8,000).  A pair that does not fit returns POYHIP_ECAP without touching the
+ * device.  Every call on one device shares that array: calls that overlap on
+ * one device (several contexts or threads) race and must be serialised by the
+ * caller. */
+int poyhip_align_affine_3_noalloc(poyhip_ctx *ctx, const uint8_t *si, int li,
+                                  const uint8_t *sj, int lj, int swaped, uint8_t *median,
+                                  int *median_len, uint8_t *medianwg, int *medianwg_len,
+                                  uint8_t *resi, uint8_t *resj, int *res_len, int cap,
+                                  int *cost);
+
 /* algn_CAML_ancestor_2 (src/algn.c:3742): median of an aligned pair with
  * POY's gap rules; cap >= len + 1. */
 int poyhip_ancestor_2(poyhip_ctx *ctx, const uint8_t *a, const uint8_t *b, int len,

@@ -78,6 +78,9 @@ SIGNATURES = {
     "poyhip_cost_affine_3": (C.c_int, [_vp, _u8p, C.c_int, _u8p, C.c_int, _ip]),
     "poyhip_align_affine_3": (C.c_int, [_vp, _u8p, C.c_int, _u8p, C.c_int, C.c_int,
                                         _u8p, _ip, _u8p, _ip, _u8p, _u8p, _ip, C.c_int, _ip]),
+    "poyhip_align_affine_3_noalloc": (C.c_int, [_vp, _u8p, C.c_int, _u8p, C.c_int, C.c_int,
+                                                _u8p, _ip, _u8p, _ip, _u8p, _u8p, _ip, C.c_int,
+                                                _ip]),
     "poyhip_ancestor_2": (C.c_int, [_vp, _u8p, _u8p, C.c_int, _u8p, C.c_int, _ip]),
     "poyhip_median_2": (C.c_int, [_vp, _u8p, _u8p, C.c_int, _u8p, C.c_int, _ip]),
     "poyhip_median_2_with_gaps": (C.c_int, [_vp, _u8p, _u8p, C.c_int, _u8p, C.c_int, _ip]),

@@ -9,6 +9,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -64,6 +65,10 @@ struct poyhip_ctx {
   bool has_cm3 = false;
   bool last3d = false;
   std::vector<uint8_t> last3[3];
+  // device address of this device's copy of `scratch` (poyhip_create looks
+  // it up; null when the lookup failed, with the reason in scratch_err)
+  uint8_t* scratch = nullptr;
+  char scratch_err[160] = {0};
 };
 
 namespace {
@@ -159,6 +164,56 @@ int stage_pair(poyhip_ctx* c, const uint8_t* a, int la, const uint8_t* b, int lb
 // Result block of the drop-ins: PairResult, 4 lengths, then the output.
 constexpr size_t kResBytes = 64;
 
+// ---- fixed device scratch of poyhip_align_affine_3_noalloc -----------------
+// One copy per device, allocated by the code-object loader and held until the
+// process ends.  Host code never uses the array's name as an address: each
+// device's copy is found once with hipGetSymbolAddress (poyhip_create) and
+// that pointer is what kernels and copies receive.
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+constexpr int kScratchDevices = 64;
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+std::mutex g_scratch_mu;
+uint8_t* g_scratch[kScratchDevices] = {};
+
+// Looks up the current device's scratch address (hipSetDevice(device) done by
+// the caller); idempotent per device.
+uint8_t* scratch_lookup(int device, char* err, size_t errn) {
+  if (device < 0 || device >= kScratchDevices) {
+    snprintf(err, errn, "device %d beyond the %d supported by the fixed scratch", device,
+             kScratchDevices);
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      snprintf(err, errn, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+      return nullptr;
+    }
+    g_scratch[device] = static_cast<uint8_t*>(p);
+  }
+  return g_scratch[device];
+}
+
+// Scratch layout of one banded affine pair: descriptor, the two sequences,
+// direction store, workspace, stripe table, result block.  Returns the bytes
+// needed; fills the offsets.
+struct AffScratch {
+  size_t seq, dm, ws, stab, out, total;
+};
+AffScratch aff_scratch_layout(int li, int lj) {
+  auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  AffScratch L;
+  L.seq = kDescBytes;
+  L.dm = up(L.seq + (size_t)li + (size_t)lj);
+  L.ws = up(L.dm + (size_t)aff_dm_bytes(li, lj));
+  L.stab = up(L.ws + (size_t)aff_ws_ints(lj) * sizeof(int));
+  L.out = up(L.stab + (size_t)(n_stripes(li) + 1) * sizeof(long long));
+  L.total = L.out + kResBytes + (size_t)aff_out_bytes(li, lj);
+  return L;
+}
+
 int run_linear_one(poyhip_ctx* c, const uint8_t* s1, int l1, const uint8_t* s2, int l2,
                    int deltawh, int prio, bool backtrace, PairResult* out_res,
                    std::vector<uint8_t>* outblk, int outlen[4]) {
@@ -234,6 +289,7 @@ int poyhip_create(int device, poyhip_ctx** out) {
     delete c;
     return POYHIP_ENOMEM;
   }
+  c->scratch = scratch_lookup(device, c->scratch_err, sizeof c->scratch_err);
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cus = prop.multiProcessorCount;
   if (c->num_cus <= 0) c->num_cus = 256;
@@ -471,6 +527,95 @@ int poyhip_align_affine_3(poyhip_ctx* c, const uint8_t* si, int li, const uint8_
   memcpy(resj, blk + 2 * cp - ol[1], ol[1]);
   memcpy(median, blk + 3 * cp - ol[2], ol[2]);
   memcpy(medianwg, blk + 4 * cp - ol[3], ol[3]);
+  *res_len = ol[0];
+  *median_len = ol[2];
+  *medianwg_len = ol[3];
+  *cost = r.cost;
+  return POYHIP_OK;
+}
+
+int poyhip_align_affine_3_noalloc(poyhip_ctx* c, const uint8_t* si, int li, const uint8_t* sj,
+                                  int lj, int swaped, uint8_t* median, int* median_len,
+                                  uint8_t* medianwg, int* medianwg_len, uint8_t* resi,
+                                  uint8_t* resj, int* res_len, int cap, int* cost) {
+  if (!c || !cost || !median || !median_len || !medianwg || !medianwg_len || !resi || !resj ||
+      !res_len || !valid_seq(si, li) || !valid_seq(sj, lj))
+    return POYHIP_EINVAL;
+  if (!c->has_cm) return set_err(c, POYHIP_ESTATE, "no cost matrix installed");
+  if (lj < li) return set_err(c, POYHIP_EINVAL, "pass the shorter one as first");
+  if (cap < li + lj + 2) return set_err(c, POYHIP_ECAP, "capacity %d < %d", cap, li + lj + 2);
+  if (!c->scratch) return set_err(c, POYHIP_EDEVICE, "no fixed scratch: %s", c->scratch_err);
+  const AffScratch L = aff_scratch_layout(li, lj);
+  if (L.total > kScratchBytes)
+    return set_err(c, POYHIP_ECAP, "pair of %d x %d needs %zu bytes of scratch, over the %zu fixed",
+                   li, lj, L.total, kScratchBytes);
+  (void)hipSetDevice(c->device);
+  uint8_t* S = c->scratch;
+  // descriptor (work counter at zero) and sequences, straight from the
+  // caller's buffers
+  OneDesc d;
+  memset(&d, 0, sizeof d);
+  d.off2 = li;
+  d.len_a = li;
+  d.len_b = lj;
+  d.prio = swaped ? 1 : 0;
+  HIPCHK(c, hipMemcpyAsync(S, &d, sizeof d, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(S + L.seq, si, (size_t)li, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(S + L.seq + li, sj, (size_t)lj, hipMemcpyHostToDevice, c->stream));
+  const PairTable t = one_table(S, S + L.seq);
+  PairResult* dres = reinterpret_cast<PairResult*>(S + L.out);
+  int* doutlen = reinterpret_cast<int*>(S + L.out + sizeof(PairResult));
+  uint8_t* dout = S + L.out + kResBytes;
+  AffFillArgs A;
+  A.pt = t;
+  A.cm = c->d_cm;
+  A.dm = S + L.dm;
+  A.ws = reinterpret_cast<int*>(S + L.ws);
+  A.ws_stride = 0;
+  A.stab = reinterpret_cast<long long*>(S + L.stab);
+  A.res = dres;
+  A.counter = &reinterpret_cast<OneDesc*>(S)->counter;
+  A.cls = nullptr;
+  A.ngen = nullptr;
+  int any = 0;
+  for (int k = 1; k < li; k++) any |= si[k];
+  for (int k = 1; k < lj; k++) any |= sj[k];
+  if (any & kGap) {
+    hipLaunchKernelGGL((k_affine_fill<true, false>), dim3(1), dim3(64), 0, c->stream, A);
+    HIPCHK(c, hipGetLastError());
+    AffBtArgs B;
+    B.pt = t;
+    B.cm = c->d_cm;
+    B.dm = A.dm;
+    B.stab = A.stab;
+    B.res = dres;
+    B.out = dout;
+    B.outlen = doutlen;
+    hipLaunchKernelGGL(k_affine_backtrace, dim3(1), dim3(64), 0, c->stream, B);
+  } else {
+    A.out = dout;
+    A.outlen = doutlen;
+    hipLaunchKernelGGL((k_affine_fill<true, true>), dim3(1), dim3(64), 0, c->stream, A);
+  }
+  HIPCHK(c, hipGetLastError());
+  // result header first, then each output region straight into the
+  // caller's buffer
+  uint8_t hdr[kResBytes];
+  HIPCHK(c, hipMemcpyAsync(hdr, S + L.out, kResBytes, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  PairResult r;
+  int ol[4];
+  memcpy(&r, hdr, sizeof r);
+  memcpy(ol, hdr + sizeof(PairResult), sizeof ol);
+  const int cp = li + lj + 2;
+  for (int q = 0; q < 4; q++)
+    if (ol[q] < 0 || ol[q] > cp) return set_err(c, POYHIP_EDEVICE, "output length %d out of range", ol[q]);
+  uint8_t* dst[4] = {resi, resj, median, medianwg};
+  for (int q = 0; q < 4; q++)
+    if (ol[q] > 0)
+      HIPCHK(c, hipMemcpyAsync(dst[q], dout + (size_t)(q + 1) * cp - ol[q], (size_t)ol[q],
+                               hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   *res_len = ol[0];
   *median_len = ol[2];
   *medianwg_len = ol[3];
