@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <numeric>
@@ -21,6 +22,7 @@
 #include "affine.hip"
 #include "cube.hip"
 #include "linear.hip"
+#include "linear_mw.hip"
 #include "medians.hip"
 
 using namespace poyhip;
@@ -65,6 +67,10 @@ struct poyhip_ctx {
   bool has_cm3 = false;
   bool last3d = false;
   std::vector<uint8_t> last3[3];
+  // the linear fill runs one workgroup of wavefronts per pair when every
+  // 5-bit cost entry fits the packed 16-bit table (set by poyhip_set_cm);
+  // POYHIP_LINEAR_FILL=wave forces the wavefront-per-pair kernel
+  bool lin_mw = false;
   // device address of this device's copy of `scratch` (poyhip_create looks
   // it up; null when the lookup failed, with the reason in scratch_err)
   uint8_t* scratch = nullptr;
@@ -214,6 +220,43 @@ AffScratch aff_scratch_layout(int li, int lj) {
   return L;
 }
 
+// Wavefronts per workgroup of the multi-wavefront linear fill and its
+// resident workgroups per CU.
+constexpr int kLinMwWaves = 8;
+constexpr int kLinMwPerCu = 2;
+
+// Workspace ints per slot that either linear fill kernel needs for columns
+// of up to Lb symbols.
+int64_t lin_ws_slot_ints(int Lb) {
+  return std::max(lin_ws_ints(Lb), lin_mw_ws_ints(Lb, kLinMwWaves));
+}
+
+// Launches the linear fill of A.pt.n pairs: the multi-wavefront kernel (one
+// workgroup per pair) when the context's table allows it, else the
+// wavefront-per-pair kernel.  `slots` workspace slots of A.ws_stride ints
+// are available.  Returns the kernel's launch error.
+hipError_t launch_linear_fill(poyhip_ctx* c, const LinFillArgs& A, bool dm, int slots,
+                              hipStream_t st) {
+  const int n = A.pt.n;
+  if (c->lin_mw) {
+    const int grid = std::max(1, std::min({n, slots, c->num_cus * kLinMwPerCu}));
+    const dim3 blk(64 * kLinMwWaves);
+    if (A.phase == 2)
+      hipLaunchKernelGGL((k_linear_mw<true, false, kLinMwWaves>), dim3(grid), blk, 0, st, A);
+    else if (dm)
+      hipLaunchKernelGGL((k_linear_mw<true, true, kLinMwWaves>), dim3(grid), blk, 0, st, A);
+    else
+      hipLaunchKernelGGL((k_linear_mw<false, true, kLinMwWaves>), dim3(grid), blk, 0, st, A);
+  } else {
+    const int grid = std::max(1, std::min(n, slots));
+    if (dm)
+      hipLaunchKernelGGL(k_linear_fill<true>, dim3(grid), dim3(64), 0, st, A);
+    else
+      hipLaunchKernelGGL(k_linear_fill<false>, dim3(grid), dim3(64), 0, st, A);
+  }
+  return hipGetLastError();
+}
+
 int run_linear_one(poyhip_ctx* c, const uint8_t* s1, int l1, const uint8_t* s2, int l2,
                    int deltawh, int prio, bool backtrace, PairResult* out_res,
                    std::vector<uint8_t>* outblk, int outlen[4]) {
@@ -224,7 +267,7 @@ int run_linear_one(poyhip_ctx* c, const uint8_t* s1, int l1, const uint8_t* s2, 
   const int64_t dmb = lin_dm_bytes(l1, l2);
   const int64_t outb = lin_out_bytes(l1, l2);
   if ((rc = grow(c, c->dm, dmb))) return rc;
-  if ((rc = grow(c, c->ws, lin_ws_ints(l2) * sizeof(int)))) return rc;
+  if ((rc = grow(c, c->ws, lin_ws_slot_ints(l2) * sizeof(int)))) return rc;
   if ((rc = grow(c, c->stab, (n_stripes(l1) + 1) * sizeof(long long)))) return rc;
   if ((rc = grow(c, c->outblk, kResBytes + outb))) return rc;
   PairResult* dres = static_cast<PairResult*>(c->outblk.p);
@@ -240,8 +283,7 @@ int run_linear_one(poyhip_ctx* c, const uint8_t* s1, int l1, const uint8_t* s2, 
   A.res = dres;
   A.counter = counter;
   A.phase = 0;
-  hipLaunchKernelGGL(k_linear_fill<true>, dim3(1), dim3(64), 0, c->stream, A);
-  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, launch_linear_fill(c, A, true, 1, c->stream));
   size_t back = sizeof(PairResult);
   if (backtrace) {
     LinBtArgs B;
@@ -356,6 +398,10 @@ int poyhip_set_cm(poyhip_ctx* c, const poyhip_cm* cm) {
   c->cm = *cm;
   c->has_cm = true;
   c->last2d = false;
+  bool fits = true;
+  for (int k = 0; k < 1024; k++) fits &= cm->cost[k] >= 0 && cm->cost[k] < 32768;
+  const char* e = getenv("POYHIP_LINEAR_FILL");
+  c->lin_mw = fits && !(e && strcmp(e, "wave") == 0);
   return POYHIP_OK;
 }
 
@@ -761,7 +807,7 @@ int64_t pair_out_bytes(int op, int la, int lb) {
 int64_t ws_ints(int op, int lb) {
   if (op == POYHIP_OP_ALIGN_AFFINE) return aff_ws_ints(lb);
   if (op == POYHIP_OP_COST_AFFINE) return affcost_ws_ints(lb);
-  return lin_ws_ints(lb);
+  return lin_ws_slot_ints(lb);
 }
 
 // Direction bytes of the final round of a linear pair with band half-width
@@ -885,10 +931,7 @@ int batch_enqueue(poyhip_batch* b, const uint8_t* blob, const int64_t* off1, con
         A.res = b->d_res;
         A.counter = b->d_counters + k;
         A.phase = 0;
-        if (b->op == POYHIP_OP_ALIGN_2D)
-          hipLaunchKernelGGL(k_linear_fill<true>, dim3(b->grid), dim3(64), 0, st, A);
-        else
-          hipLaunchKernelGGL(k_linear_fill<false>, dim3(b->grid), dim3(64), 0, st, A);
+        HIPCHK(c, launch_linear_fill(c, A, b->op == POYHIP_OP_ALIGN_2D, b->grid, st));
       }
     }
     HIPCHK(c, hipGetLastError());
@@ -1142,8 +1185,7 @@ int batch_run_two_phase(poyhip_batch* b, const uint8_t* d_blob, const int64_t* d
   A.counter = b->d_counters;
   A.phase = 1;
   HIPCHK(c, hipEventRecord(ev[1], st));
-  hipLaunchKernelGGL(k_linear_fill<false>, dim3(b->grid), dim3(64), 0, st, A);
-  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, launch_linear_fill(c, A, false, b->grid, st));
   HIPCHK(c, hipEventRecord(ev[2], st));
   b->h_res.resize(n);
   HIPCHK(c, hipMemcpyAsync(b->h_res.data(), b->d_res, sizeof(PairResult) * (size_t)n,
@@ -1179,8 +1221,7 @@ int batch_run_two_phase(poyhip_batch* b, const uint8_t* d_blob, const int64_t* d
     A.counter = b->d_counters + 1 + k;
     A.phase = 2;
     HIPCHK(c, hipEventRecord(ev[1], st));
-    hipLaunchKernelGGL(k_linear_fill<true>, dim3(std::min(b->grid, t.n)), dim3(64), 0, st, A);
-    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, launch_linear_fill(c, A, true, b->grid, st));
     HIPCHK(c, hipEventRecord(ev[2], st));
     LinBtArgs B;
     B.pt = t;

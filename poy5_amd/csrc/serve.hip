@@ -350,7 +350,7 @@ int srv_launch(poyhip_server* s, SrvSlot& sl) {
     A.counter = sl.d_counters;
     A.phase = 0;
     if (op == POYHIP_OP_ALIGN_2D) {
-      hipLaunchKernelGGL(k_linear_fill<true>, dim3(grid), dim3(64), 0, st, A);
+      HIPCHK(c, launch_linear_fill(c, A, true, grid, st));
       LinBtArgs B;
       B.pt = pt;
       B.cm = c->d_cm;
@@ -364,7 +364,7 @@ int srv_launch(poyhip_server* s, SrvSlot& sl) {
         hipLaunchKernelGGL(k_pool_store, dim3(nkeep), dim3(64), 0, st, sl.d_out, dt.out_off,
                            dt.keep_cap, sl.d_outlen, dt.keep_pid, dt.keep_pool, s->d_pool, nkeep);
     } else {
-      hipLaunchKernelGGL(k_linear_fill<false>, dim3(grid), dim3(64), 0, st, A);
+      HIPCHK(c, launch_linear_fill(c, A, false, grid, st));
     }
   }
   hipLaunchKernelGGL(k_extract_cost, dim3((n + 255) / 256), dim3(256), 0, st, sl.d_res, sl.d_cost,
@@ -498,7 +498,7 @@ int poyhip_server_create(poyhip_ctx* c, const poyhip_server_opts* o, poyhip_serv
   // outputs of a full slot: capped like the direction stores
   s->slot_out = std::max<int64_t>((int64_t)d.slot_dm_bytes / 8, aff_out_bytes(d.max_len, d.max_len) * 4);
   s->grid = std::min(d.max_batch_pairs, c->num_cus * 8);
-  s->ws_stride = (std::max(aff_ws_ints(d.max_len), lin_ws_ints(d.max_len)) + 63) & ~63LL;
+  s->ws_stride = (std::max(aff_ws_ints(d.max_len), lin_ws_slot_ints(d.max_len)) + 63) & ~63LL;
   auto fail = [&](const char* what) {
     srv_free(s);
     delete s;
