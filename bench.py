@@ -10,7 +10,9 @@ median and median-with-gaps) over every pair, inputs already resident in HBM.
 Printed JSON (one line): GCUPS on full cells (sum of L1*L2, leading gaps
 included) per second of the timed steps, the fill kernel's VALU roofline,
 the plain per-call GPU path (one launch per pair through the drop-in call)
-on a sample, and the CPU oracle on a bounded sample of host cores.
+on a sample, the CPU oracle on a bounded sample over the host cores, and
+the other configurations (1, 4, 5 at full size, each with its roofline and
+CPU baseline) plus the serving run.
 """
 from __future__ import annotations
 
@@ -38,8 +40,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--pairs", type=int, default=65536)
     ap.add_argument("--plain-sample", type=int, default=512)
-    ap.add_argument("--cpu-stride", type=int, default=16,
-                    help="CPU baseline runs every N-th pair")
+    ap.add_argument("--cpu-stride", type=int, default=4,
+                    help="CPU baseline of the headline runs every N-th pair")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-serve", action="store_true",
@@ -47,8 +49,9 @@ def parse():
     ap.add_argument("--serve-requests", type=int, default=4000)
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the secondary configurations (1, 4, 5)")
-    ap.add_argument("--extra-long-pairs", type=int, default=512)
-    ap.add_argument("--extra-triples", type=int, default=1024)
+    ap.add_argument("--c1-pairs", type=int, default=1024)
+    ap.add_argument("--c4-pairs", type=int, default=2048)
+    ap.add_argument("--c5-triples", type=int, default=16384)
     ap.add_argument("--graph", type=int, default=0,
                     help="replay each step as one HIP graph (no per-kernel events then)")
     ap.add_argument("--dm-budget-gb", type=float, default=0.0)
@@ -57,61 +60,95 @@ def parse():
     return ap.parse_args()
 
 
-_CPU_DATA = None  # (blob, off, ln), inherited by the forked workers
+def host_cores():
+    """Cores this process may use, capped at the 16 a one-GPU box allots."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
 
 
-def cpu_worker(idx):
-    """One host process of the CPU baseline: the oracle over its share."""
-    blob, off, ln = _CPU_DATA
+def cpu_job(job):
+    """One host process of a CPU baseline: the oracle over its share of a
+    sample.  job = (kind, tcm, blob, offsets..., lengths...)."""
     import numpy as np
     import oracle
     L = oracle.lib()
-    cm = oracle.cm_dna(1, 1, 3)
-    off1 = np.ascontiguousarray(off[0::2][idx])
-    off2 = np.ascontiguousarray(off[1::2][idx])
-    l1 = np.ascontiguousarray(ln[0::2][idx]).astype(np.int32)
-    l2 = np.ascontiguousarray(ln[1::2][idx]).astype(np.int32)
-    costs = np.zeros(len(idx), np.int32)
-    cells = C.c_int64()
-    t0 = time.perf_counter()
-    L.orc_batch_align_affine(len(idx), blob.ctypes.data_as(C.POINTER(C.c_uint8)),
-                             off1.ctypes.data_as(C.POINTER(C.c_int64)),
-                             l1.ctypes.data_as(C.POINTER(C.c_int)),
-                             off2.ctypes.data_as(C.POINTER(C.c_int64)),
-                             l2.ctypes.data_as(C.POINTER(C.c_int)), C.byref(cm),
-                             costs.ctypes.data_as(C.POINTER(C.c_int)), C.byref(cells))
-    return time.perf_counter() - t0, costs, idx
+    kind = job[0]
+    P = lambda a, t: a.ctypes.data_as(C.POINTER(t))  # noqa: E731
+    if kind == "affine":
+        _, tcm, blob, o1, l1, o2, l2 = job
+        cm = oracle.cm_dna(*tcm)
+        costs = np.zeros(len(l1), np.int32)
+        cells = C.c_int64()
+        t0 = time.perf_counter()
+        L.orc_batch_align_affine(len(l1), P(blob, C.c_uint8), P(o1, C.c_int64), P(l1, C.c_int),
+                                 P(o2, C.c_int64), P(l2, C.c_int), C.byref(cm),
+                                 P(costs, C.c_int), C.byref(cells))
+    elif kind == "linear":
+        _, tcm, blob, o1, l1, o2, l2 = job
+        cm = oracle.cm_dna(*tcm)
+        costs = np.zeros(len(l1), np.int32)
+        cells = C.c_int64()
+        t0 = time.perf_counter()
+        L.orc_batch_align_2(len(l1), P(blob, C.c_uint8), P(o1, C.c_int64), P(l1, C.c_int),
+                            P(o2, C.c_int64), P(l2, C.c_int), C.byref(cm), P(costs, C.c_int),
+                            C.byref(cells))
+    else:
+        _, tcm, blob, o1, l1, o2, l2, o3, l3 = job
+        c3 = oracle.cm3_from_2d(oracle.cm_dna(*tcm))
+        costs = np.zeros(len(l1), np.int32)
+        t0 = time.perf_counter()
+        L.orc_batch_align_3d(len(l1), P(blob, C.c_uint8), P(o1, C.c_int64), P(l1, C.c_int),
+                             P(o2, C.c_int64), P(l2, C.c_int), P(o3, C.c_int64), P(l3, C.c_int),
+                             C.byref(c3), P(costs, C.c_int))
+    return time.perf_counter() - t0, costs
 
 
-def cpu_baseline(blob, off, ln, stride, gpu_costs):
-    """The oracle (a C restatement of POY's algn.c path) on every stride-th
-    pair, one process per host core (16 at most), LPT-partitioned."""
-    import multiprocessing as mp
+def cpu_baseline(pool, procs, kind, tcm, blob, offs, lens, sample, gpu_costs, what):
+    """The oracle (a C restatement of POY's algn.c path) on the pairs or
+    triples `sample`, LPT-partitioned over `procs` worker processes (forked
+    before any GPU work).  GCUPS on full cells."""
     import numpy as np
-    n = len(ln) // 2
-    sample = np.arange(0, n, stride)
-    work = ln[0::2][sample].astype(np.float64) * ln[1::2][sample]
-    cores = max(1, min(16, os.cpu_count() or 1))
-    parts = [[] for _ in range(cores)]
-    load = [0.0] * cores
+    nseq = len(offs)
+    work = np.ones(len(sample))
+    for u in range(nseq):
+        work = work * lens[u][sample].astype(np.float64)
+    parts = [[] for _ in range(procs)]
+    load = [0.0] * procs
     for k in np.argsort(-work):
-        p = int(np.argmin(load))
-        parts[p].append(sample[k])
-        load[p] += work[k]
-    global _CPU_DATA
-    _CPU_DATA = (blob, off, ln)
-    jobs = [np.array(p, np.int64) for p in parts if p]
-    ctx = mp.get_context("fork")
+        q = int(np.argmin(load))
+        parts[q].append(k)
+        load[q] += work[k]
+    jobs, idx = [], []
+    for part in parts:
+        if not part:
+            continue
+        sel = sample[np.array(part, np.int64)]
+        # pack the share's sequences into one blob
+        pieces, newoffs, o = [], [[] for _ in range(nseq)], 0
+        for k in sel:
+            for u in range(nseq):
+                seq = blob[offs[u][k]: offs[u][k] + lens[u][k]]
+                newoffs[u].append(o)
+                pieces.append(seq)
+                o += len(seq)
+        sub = np.concatenate(pieces).astype(np.uint8)
+        args = [kind, tcm, sub]
+        for u in range(nseq):
+            args += [np.array(newoffs[u], np.int64), lens[u][sel].astype(np.int32)]
+        jobs.append(tuple(args))
+        idx.append(sel)
     t0 = time.perf_counter()
-    with ctx.Pool(len(jobs)) as pool:
-        res = pool.map(cpu_worker, jobs)
+    res = pool.map(cpu_job, jobs)
     wall = time.perf_counter() - t0
-    match = all(np.array_equal(c, gpu_costs[i]) for _, c, i in res)
+    match = all(np.array_equal(c, gpu_costs[i]) for (_, c), i in zip(res, idx))
     cells = float(work.sum())
-    return {"value": cells / wall / 1e9, "unit": "GCUPS", "cores": len(jobs), "kind": "port",
-            "sample": f"every {stride}th pair of the workload ({len(sample)} pairs, "
-                      f"{cells:.3e} full cells), oracle/oracle.c compiled -O2, "
-                      f"{len(jobs)} processes, wall {wall:.2f} s",
+    return {"value": round(cells / wall / 1e9, 3), "unit": "GCUPS", "cores": len(jobs),
+            "kind": "port",
+            "sample": f"{what} ({len(sample)} of {len(lens[0])}, {cells:.3e} full cells), "
+                      f"oracle/oracle.c compiled -O2, {len(jobs)} processes, wall {wall:.2f} s",
             "costs_match_gpu": bool(match)}
 
 
@@ -154,12 +191,18 @@ def _timed(run, steps=2):
     return (time.perf_counter() - t0) / steps
 
 
-def secondary(args):
-    """GCUPS of the other workloads on full cells, batched, inputs resident:
+# int32 adds and compares per cell of the reference recurrence
+OPS_LINEAR = 6    # algn_fill_row: 3 candidates x (add + compare)
+OPS_CUBE = 14     # algn_fill_cube: 7 candidates x (add + compare)
+
+
+def secondary(args, pool, procs):
+    """The other workloads at their full sizes, batched, inputs resident:
     config 1 (1,024 linear pairs, align_2 + ancestor + median with gaps),
-    config 4 (the first N pairs of 16k-100k, linear sub 1 / indel 1, band
-    doubling, align_2 + medians), config 5 (the first N triples, 3D cube +
-    backtrack + median_3)."""
+    config 4 (2,048 pairs of 16k-100k, linear sub 1 / indel 1, band
+    doubling, align_2 + medians), config 5 (16,384 triples, 3D cube +
+    backtrack + median_3).  GCUPS on full cells; roofline of the fill stage
+    on the cells the reference's loops evaluate; the oracle on a sample."""
     import numpy as np
     import torch
     from poy5_amd import cost_matrix, workloads
@@ -167,7 +210,10 @@ def secondary(args):
     from poy5_amd.sequence import Context
     dev = torch.device("cuda:0")
     out = {}
-    for cfg, n, tcm in ((1, 1024, (1, 2)), (4, args.extra_long_pairs, (1, 1))):
+    for cfg, n, tcm, steps, cpu_stride, cpu_procs in (
+            (1, args.c1_pairs, (1, 2), 3, 1, procs),
+            (4, args.c4_pairs, (1, 1), 1, 32, max(1, procs // 2))):
+        print(f"[bench] config {cfg}: {n} pairs", file=sys.stderr, flush=True)
         ctx = Context(0)
         ctx.set_cm(cost_matrix.of_transformations_and_gaps(*tcm))
         blob, off, ln = workloads.gen_pairs(cfg, n)
@@ -177,21 +223,43 @@ def secondary(args):
         d1 = torch.from_numpy(np.ascontiguousarray(off[0::2])).to(dev)
         d2 = torch.from_numpy(np.ascontiguousarray(off[1::2])).to(dev)
         cost, o, ol = b.alloc_outputs(dev)
-        dt = _timed(lambda: b.run(d_blob, d1, d2, cost, o, ol, sync=True))
+        fills = []
+
+        def run():
+            b.run(d_blob, d1, d2, cost, o, ol, sync=True)
+            fills.append(b.fill_ms())
+        dt = _timed(run, steps=steps)
+        fill_ms = float(np.mean(fills[1:]))
         cells = float(np.sum(l1.astype(np.float64) * l2))
-        cb, _, rounds, _ = b.stats()
-        out[f"config{cfg}"] = {"pairs": n, "op": "align_2d", "tcm": list(tcm),
-                               "ms": round(dt * 1e3, 3), "cells_full": cells,
-                               "cells_computed": float(cb.sum()),
-                               "gcups": round(cells / dt / 1e9, 2),
-                               "chunks": b.chunks(), "mean_rounds": float(rounds.mean())}
+        cb, sl, rounds, _ = b.stats()
+        computed = float(cb.sum())
+        achieved = computed * OPS_LINEAR / (fill_ms / 1e3) / 1e12
+        rec = {"pairs": n, "op": "align_2d", "tcm": list(tcm), "ms": round(dt * 1e3, 3),
+               "cells_full": cells, "cells_computed": computed,
+               "gcups": round(cells / dt / 1e9, 2), "chunks": b.chunks(),
+               "mean_rounds": float(rounds.mean()),
+               "roofline": {"bound": "valu", "achieved": round(achieved, 3),
+                            "peak": VALU_PEAK_TOPS, "unit": "TOP/s",
+                            "frac": round(achieved / VALU_PEAK_TOPS, 4),
+                            "ops_per_cell": OPS_LINEAR, "fill_ms": round(fill_ms, 3),
+                            "kernel": "k_linear_mw (fill stage, every doubling round)"}}
+        gpu_costs = cost.cpu().numpy()
         b.close()
+        del cost, o, ol, d_blob
         ctx.close()
+        torch.cuda.empty_cache()
+        if pool is not None:
+            sample = np.arange(0, n, cpu_stride)
+            rec["cpu_baseline"] = cpu_baseline(
+                pool, cpu_procs, "linear", tcm, blob, [off[0::2], off[1::2]], [l1, l2],
+                sample, gpu_costs, f"every {cpu_stride}th pair" if cpu_stride > 1 else "all pairs")
+        out[f"config{cfg}"] = rec
+    print(f"[bench] config 5: {args.c5_triples} triples", file=sys.stderr, flush=True)
     ctx = Context(0)
     two = cost_matrix.of_transformations_and_gaps(1, 1)
     ctx.set_cm(two)
     ctx.set_cm3(cost_matrix.three_d_of_two_dim_comb(two))
-    n3 = args.extra_triples
+    n3 = args.c5_triples
     blob, off, ln = workloads.gen_triples(n3)
     lens = [ln[u::3].astype(np.int32) for u in range(3)]
     b3 = Batch3(ctx, "align_3d", *lens)
@@ -200,11 +268,26 @@ def secondary(args):
     cost, o, ol = b3.alloc_outputs(dev)
     dt = _timed(lambda: b3.run(d_blob, *ds, cost, o, ol, sync=True), steps=1)
     cells = float(np.sum(lens[0].astype(np.float64) * lens[1] * lens[2]))
-    out["config5"] = {"triples": n3, "op": "align_3d", "ms": round(dt * 1e3, 3),
-                      "cells_full": cells, "gcups": round(cells / dt / 1e9, 2),
-                      "chunks": b3.chunks(), "fill_ms": round(b3.fill_ms(), 3)}
+    fill_ms = b3.fill_ms()
+    achieved = cells * OPS_CUBE / (fill_ms / 1e3) / 1e12
+    rec = {"triples": n3, "op": "align_3d", "ms": round(dt * 1e3, 3), "cells_full": cells,
+           "gcups": round(cells / dt / 1e9, 2), "chunks": b3.chunks(),
+           "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": VALU_PEAK_TOPS,
+                        "unit": "TOP/s", "frac": round(achieved / VALU_PEAK_TOPS, 4),
+                        "ops_per_cell": OPS_CUBE, "fill_ms": round(fill_ms, 3),
+                        "kernel": "k_cube_fill (fill stage)"}}
+    gpu_costs = cost.cpu().numpy()
     b3.close()
+    del cost, o, ol, d_blob
     ctx.close()
+    torch.cuda.empty_cache()
+    if pool is not None:
+        stride = 1024
+        sample = np.arange(0, n3, stride)
+        rec["cpu_baseline"] = cpu_baseline(
+            pool, max(1, procs // 2), "cube", (1, 1), blob, [off[0::3], off[1::3], off[2::3]],
+            lens, sample, gpu_costs, f"every {stride}th triple")
+    out["config5"] = rec
     return out
 
 
@@ -251,6 +334,21 @@ def main():
         print(f"bench.py measures one GPU: this build runs on a single MI355X and "
               f"has no multi-GPU path (asked for --gpus {args.gpus}).", file=sys.stderr)
         sys.exit(2)
+    # the CPU baseline's worker processes are forked first, from a process
+    # that has touched neither the GPU nor the HIP runtime
+    pool, procs = None, host_cores()
+    if not args.no_cpu:
+        import multiprocessing as mp
+        pool = mp.get_context("fork").Pool(procs)
+    try:
+        run_bench(args, pool, procs)
+    finally:
+        if pool is not None:
+            pool.close()
+            pool.join()
+
+
+def run_bench(args, pool, procs):
     import numpy as np
     import torch
 
@@ -364,15 +462,19 @@ def main():
         result["batched_vs_plain"] = round(gcups / pl["gcups"], 2)
         result["p50_ms"] = round(pl["p50_ms"], 3)
         result["p99_ms"] = round(pl["p99_ms"], 3)
+    if pool is not None:
+        print("[bench] CPU baseline (config 2 sample)", file=sys.stderr, flush=True)
+        sample = np.arange(0, args.pairs, args.cpu_stride)
+        result["cpu_baseline"] = cpu_baseline(
+            pool, procs, "affine", (1, 1, 3), blob, [off[0::2], off[1::2]], [len1, len2], sample,
+            gpu_costs, f"every {args.cpu_stride}th pair of the workload")
+        result["cpu_baseline"]["host_cpus"] = os.cpu_count()
     if not args.no_extra:
         print("[bench] secondary configurations", file=sys.stderr, flush=True)
-        result["secondary"] = secondary(args)
+        result["secondary"] = secondary(args, pool, procs)
     if not args.no_serve:
         print("[bench] serving: 128-taxon SPR rejoin traffic", file=sys.stderr, flush=True)
         result["serving"] = serving(args.serve_requests)
-    if not args.no_cpu:
-        print("[bench] CPU baseline", file=sys.stderr, flush=True)
-        result["cpu_baseline"] = cpu_baseline(blob, off, ln, args.cpu_stride, gpu_costs)
     if args.copy_calib_gb > 0:
         nb = int(args.copy_calib_gb * (1 << 30))
         x = torch.ones(nb, dtype=torch.uint8, device=dev)

@@ -93,6 +93,14 @@ def lib():
                                              C.POINTER(C.c_int64), ip, C.POINTER(OrcCM), ip,
                                              C.POINTER(C.c_int64)]
         L.orc_batch_align_affine.restype = C.c_int64
+        L.orc_batch_align_2.argtypes = [C.c_int, u8p, C.POINTER(C.c_int64), ip,
+                                        C.POINTER(C.c_int64), ip, C.POINTER(OrcCM), ip,
+                                        C.POINTER(C.c_int64)]
+        L.orc_batch_align_2.restype = C.c_int64
+        L.orc_batch_align_3d.argtypes = [C.c_int, u8p, C.POINTER(C.c_int64), ip,
+                                         C.POINTER(C.c_int64), ip, C.POINTER(C.c_int64), ip,
+                                         C.POINTER(OrcCM3), ip]
+        L.orc_batch_align_3d.restype = C.c_int64
         _lib = L
     return _lib
 

@@ -1340,3 +1340,52 @@ int64_t orc_batch_align_affine(int n, const uint8_t *blob, const int64_t *off1,
     if (cells_out) *cells_out = cells;
     return total;
 }
+
+/* Align.align_2 + ancestor_2 + median_2_with_gaps per pair (the linear batch
+ * op); returns the sum of costs.  CPU baseline helper. */
+int64_t orc_batch_align_2(int n, const uint8_t *blob, const int64_t *off1, const int *len1,
+                          const int64_t *off2, const int *len2, const orc_cm *cm, int *costs,
+                          int64_t *cells_out) {
+    int64_t total = 0, cells = 0;
+    for (int k = 0; k < n; k++) {
+        const uint8_t *a = blob + off1[k], *b = blob + off2[k];
+        int la = len1[k], lb = len2[k];
+        int cap = la + lb + 1;
+        uint8_t *buf = (uint8_t *)malloc((size_t)cap * 4);
+        int al = 0;
+        orc_stats st;
+        int c = orc_align_2(a, la, b, lb, cm, buf, buf + cap, &al, &st);
+        orc_ancestor_2(buf, buf + cap, al, cm, buf + 2 * cap);
+        orc_median_2_with_gaps(buf, buf + cap, al, cm, buf + 3 * cap);
+        free(buf);
+        if (costs) costs[k] = c;
+        total += c;
+        cells += st.cells;
+    }
+    if (cells_out) *cells_out = cells;
+    return total;
+}
+
+/* algn_nw_3d + backtrack_3d + median_3 per triple; returns the sum of costs.
+ * CPU baseline helper. */
+int64_t orc_batch_align_3d(int n, const uint8_t *blob, const int64_t *off1, const int *len1,
+                           const int64_t *off2, const int *len2, const int64_t *off3,
+                           const int *len3, const orc_cm3 *c3, int *costs) {
+    int64_t total = 0;
+    for (int k = 0; k < n; k++) {
+        const uint8_t *a = blob + off1[k], *b = blob + off2[k], *c = blob + off3[k];
+        int l1 = len1[k], l2 = len2[k], l3 = len3[k];
+        size_t cells = (size_t)l1 * l2 * l3;
+        uint16_t *dm = (uint16_t *)malloc(sizeof(uint16_t) * (cells ? cells : 1));
+        int cost = orc_nw_3d(a, l1, b, l2, c, l3, c3, dm);
+        int cap = l1 + l2 + l3;
+        uint8_t *buf = (uint8_t *)malloc((size_t)cap * 4);
+        int len = orc_backtrack_3d(a, l1, b, l2, c, l3, dm, GAP, buf, buf + cap, buf + 2 * cap);
+        if (len > 0) orc_median_3(buf, buf + cap, buf + 2 * cap, len, c3, buf + 3 * cap);
+        free(buf);
+        free(dm);
+        if (costs) costs[k] = cost;
+        total += cost;
+    }
+    return total;
+}

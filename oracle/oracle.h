@@ -183,6 +183,16 @@ int64_t orc_batch_align_affine(int n, const uint8_t *blob, const int64_t *off1,
                                const int *len2, const orc_cm *cm,
                                int *costs, int64_t *cells_out);
 
+/* Align.align_2 + ancestor_2 + median_2_with_gaps per pair (caller order);
+ * returns the sum of costs. */
+int64_t orc_batch_align_2(int n, const uint8_t *blob, const int64_t *off1, const int *len1,
+                          const int64_t *off2, const int *len2, const orc_cm *cm, int *costs,
+                          int64_t *cells_out);
+/* algn_nw_3d + backtrack_3d + median_3 per triple; returns the sum of costs. */
+int64_t orc_batch_align_3d(int n, const uint8_t *blob, const int64_t *off1, const int *len1,
+                           const int64_t *off2, const int *len2, const int64_t *off3,
+                           const int *len3, const orc_cm3 *c3, int *costs);
+
 #ifdef __cplusplus
 }
 #endif
