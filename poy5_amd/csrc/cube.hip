@@ -25,8 +25,10 @@
 // its 16 cells into a map y -> min(a, y + b), a Kogge-Stone scan over the
 // wavefront composes the maps, and a second pass applies the incoming value.
 //
-// Directions: one nibble per cell (1 P1, 2 P2, 3 P3, 4 S1, 5 S2, 6 S3,
-// 7 SS), 512 bytes per row segment, nibble order = cell order.
+// Directions: one nibble per cell, 512 bytes per row segment, nibble order =
+// cell order.  The codes number the moves in the order they are tried
+// (1 P3, 2 P1, 3 P2, 4 S3, 5 S1, 6 S2, 7 SS), so that among equal values the
+// first move tried has the smallest code.
 #pragma once
 #include "poyhip_device.h"
 #include "poyhip_layout.h"
@@ -40,7 +42,7 @@ struct Cm3Dev {
   uint8_t median[32768];
 };
 
-enum : int { kD3P1 = 1, kD3P2 = 2, kD3P3 = 3, kD3S1 = 4, kD3S2 = 5, kD3S3 = 6, kD3SS = 7 };
+enum : int { kD3P3 = 1, kD3P1 = 2, kD3P2 = 3, kD3S3 = 4, kD3S1 = 5, kD3S2 = 6, kD3SS = 7 };
 
 constexpr int kCubeCpl = 16;                  // cells per lane per segment
 constexpr int kCubeSeg = 64 * kCubeCpl;       // cells per segment
@@ -487,7 +489,7 @@ __global__ __launch_bounds__(64) void k_cube_backtrack(CubeBtArgs A) {
     const int di = (d == kD3S2 || d == kD3S3 || d == kD3P3 || d == kD3P2);
     const int dj = (d == kD3S2 || d == kD3S1 || d == kD3P1 || d == kD3P2);
     const int dk = (d == kD3S2 || d == kD3S3 || d == kD3S1 || d == kD3SS);
-    if (d < kD3P1 || d > kD3SS || i - di < 0 || j - dj < 0 || k - dk < 0) { bad = 1; break; }
+    if (d < kD3P3 || d > kD3SS || i - di < 0 || j - dj < 0 || k - dk < 0) { bad = 1; break; }
     const int p = cap - 1 - n;
     r1[p] = di ? s1[i] : kGap;
     r2[p] = dj ? s2[j] : kGap;

@@ -21,6 +21,7 @@
 
 #include "affine.hip"
 #include "cube.hip"
+#include "cube_mw.hip"
 #include "linear.hip"
 #include "linear_mw.hip"
 #include "medians.hip"
@@ -71,6 +72,12 @@ struct poyhip_ctx {
   // 5-bit cost entry fits the packed 16-bit table (set by poyhip_set_cm);
   // POYHIP_LINEAR_FILL=wave forces the wavefront-per-pair kernel
   bool lin_mw = false;
+  // one-segment triples (L3 <= 1024) fill with one workgroup of wavefronts
+  // per triple; POYHIP_CUBE_FILL=wave forces the wavefront-per-triple kernel
+  bool cube_mw = true;
+  // largest entry of the 3D cost table, -1 when an entry is negative (the
+  // workgroup fill carries 8*value + code keys and needs values below 2^27)
+  int cm3_max = -1;
   // device address of this device's copy of `scratch` (poyhip_create looks
   // it up; null when the lookup failed, with the reason in scratch_err)
   uint8_t* scratch = nullptr;
@@ -227,6 +234,13 @@ constexpr int kLinMwPerCu = 2;
 
 // Workspace ints per slot that either linear fill kernel needs for columns
 // of up to Lb symbols.
+// The workgroup cube fill takes a one-segment triple when its keys cannot
+// wrap: every value of the cube is at most (L1 + L2 + L3) times the largest
+// table entry.
+bool cube_mw_ok(const poyhip_ctx* c, int64_t len_sum) {
+  return c->cube_mw && c->cm3_max >= 0 && (len_sum + 1) * (int64_t)c->cm3_max < ((int64_t)1 << 27);
+}
+
 int64_t lin_ws_slot_ints(int Lb) {
   return std::max(lin_ws_ints(Lb), lin_mw_ws_ints(Lb, kLinMwWaves));
 }
@@ -335,6 +349,8 @@ int poyhip_create(int device, poyhip_ctx** out) {
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cus = prop.multiProcessorCount;
   if (c->num_cus <= 0) c->num_cus = 256;
+  const char* e = getenv("POYHIP_CUBE_FILL");
+  c->cube_mw = !(e && strcmp(e, "wave") == 0);
   *out = c;
   return POYHIP_OK;
 }
@@ -1371,7 +1387,8 @@ int run_cube_one(poyhip_ctx* c, const uint8_t* s1, int l1, const uint8_t* s2, in
   const int64_t dmb = cube_dm_bytes(l1, l2, l3);
   const int64_t outb = cube_out_bytes(l1, l2, l3);
   if ((rc = grow(c, c->dm, dmb + 256))) return rc;
-  if ((rc = grow(c, c->ws, cube_ring_ints(l1, l2, l3) * sizeof(int)))) return rc;
+  const int64_t ring = std::max(cube_ring_ints(l1, l2, l3), cube_mw_ring_ints(l1, l2, l3, kCubeMwWaves));
+  if ((rc = grow(c, c->ws, ring * sizeof(int)))) return rc;
   if ((rc = grow(c, c->outblk, kResBytes + outb))) return rc;
   c->h_stage.resize(need);
   OneDesc3 d;
@@ -1399,7 +1416,14 @@ int run_cube_one(poyhip_ctx* c, const uint8_t* s1, int l1, const uint8_t* s2, in
   A.ring_stride = 0;
   A.cost = dcost;
   A.counter = &reinterpret_cast<OneDesc3*>(c->stage.p)->counter;
-  if (cube_nseg(l3) == 1)
+  const int64_t lsum = (int64_t)l1 + l2 + l3;
+  if (cube_nseg(l3) == 1 && cube_mw_ok(c, lsum) && cube_mw_u16(lsum, c->cm3_max))
+    hipLaunchKernelGGL((k_cube_mw<true, true, kCubeMwWaves>), dim3(1), dim3(64 * kCubeMwWaves), 0,
+                       c->stream, A);
+  else if (cube_nseg(l3) == 1 && cube_mw_ok(c, lsum))
+    hipLaunchKernelGGL((k_cube_mw<true, false, kCubeMwWaves>), dim3(1), dim3(64 * kCubeMwWaves),
+                       0, c->stream, A);
+  else if (cube_nseg(l3) == 1)
     hipLaunchKernelGGL((k_cube_fill<true, true>), dim3(1), dim3(64), 0, c->stream, A);
   else
     hipLaunchKernelGGL((k_cube_fill<true, false>), dim3(1), dim3(64), 0, c->stream, A);
@@ -1462,6 +1486,9 @@ int poyhip_set_cm3(poyhip_ctx* c, const poyhip_cm3* cm3) {
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->has_cm3 = true;
   c->last3d = false;
+  int mx = 0;
+  for (int k = 0; k < 32768 && mx >= 0; k++) mx = cm3->cost[k] < 0 ? -1 : std::max(mx, cm3->cost[k]);
+  c->cm3_max = mx;
   return POYHIP_OK;
 }
 
@@ -1569,6 +1596,7 @@ struct poyhip_batch3 {
   int64_t dm_cap = 0;
   long long ring_stride = 0;
   int grid = 0;
+  int64_t len_sum_1seg = 0;  // largest L1 + L2 + L3 among one-segment triples
   size_t out_bytes = 0;
   std::vector<hipEvent_t> ev;
   float fill_ms = -1.f, total_ms = -1.f;
@@ -1615,7 +1643,11 @@ int poyhip_batch3_create(poyhip_ctx* c, int op, int n, const int* len1, const in
       return set_err(c, POYHIP_EINVAL, "triple %d has an empty sequence", k);
     }
     work[k] = (double)len1[k] * len2[k] * len3[k];
+    if (cube_nseg(len3[k]) == 1)
+      b->len_sum_1seg = std::max<int64_t>(b->len_sum_1seg, (int64_t)len1[k] + len2[k] + len3[k]);
     b->ring_stride = std::max<long long>(b->ring_stride, cube_ring_ints(len1[k], len2[k], len3[k]));
+    b->ring_stride = std::max<long long>(
+        b->ring_stride, cube_mw_ring_ints(len1[k], len2[k], len3[k], kCubeMwWaves));
   }
   b->ring_stride = (b->ring_stride + 63) & ~63LL;
   b->order.resize(n);
@@ -1763,12 +1795,29 @@ int poyhip_batch3_run(poyhip_batch3* b, const uint8_t* d_blob, const int64_t* d_
     A.counter = b->d_counters + k;
     const int g = std::min(b->grid, ch.count);
     HIPCHK(c, hipEventRecord(b->ev[2 * k], st));
-    // one launch per row shape (counters 2k and 2k+1)
+    // one launch per row shape (counters 2k and 2k+1): one-segment triples
+    // on workgroups of kCubeMwWaves wavefronts (two per CU), the longer ones
+    // on single wavefronts
     A.counter = b->d_counters + 2 * k;
-    if (b->op == POYHIP_OP_ALIGN_3D)
+    const bool dm = b->op == POYHIP_OP_ALIGN_3D;
+    if (cube_mw_ok(c, b->len_sum_1seg)) {
+      const int gm = std::min(g, c->num_cus * 2);
+      const dim3 blk(64 * kCubeMwWaves);
+      if (cube_mw_u16(b->len_sum_1seg, c->cm3_max)) {
+        if (dm)
+          hipLaunchKernelGGL((k_cube_mw<true, true, kCubeMwWaves>), dim3(gm), blk, 0, st, A);
+        else
+          hipLaunchKernelGGL((k_cube_mw<false, true, kCubeMwWaves>), dim3(gm), blk, 0, st, A);
+      } else if (dm) {
+        hipLaunchKernelGGL((k_cube_mw<true, false, kCubeMwWaves>), dim3(gm), blk, 0, st, A);
+      } else {
+        hipLaunchKernelGGL((k_cube_mw<false, false, kCubeMwWaves>), dim3(gm), blk, 0, st, A);
+      }
+    } else if (dm) {
       hipLaunchKernelGGL((k_cube_fill<true, true>), dim3(g), dim3(64), 0, st, A);
-    else
+    } else {
       hipLaunchKernelGGL((k_cube_fill<false, true>), dim3(g), dim3(64), 0, st, A);
+    }
     A.counter = b->d_counters + 2 * k + 1;
     if (b->op == POYHIP_OP_ALIGN_3D)
       hipLaunchKernelGGL((k_cube_fill<true, false>), dim3(g), dim3(64), 0, st, A);

@@ -112,3 +112,86 @@ def test_batch_3d_config5_sample(env):
         assert L(outs[k][3]) == L(oracle.median_3(*bt, c3))
     c2, _ = batch.run_triples(ctx, "cost_3d", trip)
     assert np.array_equal(c2, costs)
+
+
+def test_batch_3d_workgroup_fill_equals_wavefront_fill(env):
+    """The workgroup-per-triple fill (one segment rows) against the
+    wavefront-per-triple fill on config-5 triples, and the oracle on the two
+    smallest: costs, aligned triples and medians identical."""
+    from poy5_amd import batch, workloads
+    from poy5_amd import cost_matrix
+    from poy5_amd.sequence import Context
+    ctx, al, c3 = env
+    blob, off, ln = workloads.gen_triples(256)
+    vol = ln[0::3].astype(np.float64) * ln[1::3] * ln[2::3]
+    pick = np.argsort(vol)[:24]
+    trip = [tuple(blob[off[3 * k + u]: off[3 * k + u] + ln[3 * k + u]] for u in range(3))
+            for k in pick]
+    rng = np.random.default_rng(34)
+    for _ in range(24):
+        n = rng.integers(1, 90, 3)
+        trip.append(tuple(rseq(rng, int(x) - 1) for x in n))
+    costs, outs = batch.run_triples(ctx, "align_3d", trip)
+    os.environ["POYHIP_CUBE_FILL"] = "wave"
+    try:
+        wctx = Context(0)
+    finally:
+        del os.environ["POYHIP_CUBE_FILL"]
+    try:
+        two = cost_matrix.of_transformations_and_gaps(1, 1)
+        wctx.set_cm(two)
+        wctx.set_cm3(cost_matrix.three_d_of_two_dim_comb(two))
+        wcosts, wouts = batch.run_triples(wctx, "align_3d", trip)
+        wc2, _ = batch.run_triples(wctx, "cost_3d", trip)
+    finally:
+        wctx.close()
+    assert np.array_equal(costs, wcosts)
+    assert np.array_equal(wc2, costs)
+    for k in range(len(trip)):
+        assert [L(x) for x in outs[k]] == [L(x) for x in wouts[k]], k
+    for k in range(2):
+        s1, s2, s3 = trip[k]
+        want, dm = oracle.nw_3d(s1, s2, s3, c3)
+        assert costs[k] == want
+        bt = oracle.backtrack_3d(s1, s2, s3, dm)
+        assert [L(x) for x in outs[k][:3]] == [L(x) for x in bt]
+
+
+@pytest.mark.parametrize("tcm", [(5, 7), (1000000, 1500000)])
+def test_cube_ring_widths_and_fallback(tcm):
+    """Cost models whose cube values need 32-bit ring rows or keys too wide
+    for the workgroup fill (which then falls back to the wavefront fill):
+    sub 5 / indel 7 keeps 16-bit rows only while L1+L2+L3 < 666, sub 10^6 /
+    indel 1.5*10^6 leaves the workgroup fill below L1+L2+L3 = 52.  Drop-in
+    calls decide per triple, batches per batch; all against the oracle."""
+    from poy5_amd import batch, cost_matrix
+    from poy5_amd.sequence import Align, Context
+    ctx = Context(0)
+    try:
+        two = cost_matrix.of_transformations_and_gaps(*tcm)
+        ctx.set_cm(two)
+        ctx.set_cm3(cost_matrix.three_d_of_two_dim_comb(two))
+        al = Align(ctx)
+        c3 = oracle.cm3_from_2d(oracle.cm_dna(*tcm))
+        rng = np.random.default_rng(35)
+        if tcm == (5, 7):
+            small = [tuple(int(x) for x in rng.integers(5, 200, 3)) for _ in range(10)]
+            large = [(300, 250, 330), (180, 420, 260)]
+        else:
+            small = [tuple(int(x) for x in rng.integers(3, 17, 3)) for _ in range(10)]
+            large = [(20, 30, 25), (33, 12, 40)]
+        for shapes in (small, small + large):
+            trip = [tuple(rseq(rng, n - 1) for n in sh) for sh in shapes]
+            costs, outs = batch.run_triples(ctx, "align_3d", trip)
+            for k, (s1, s2, s3) in enumerate(trip):
+                want, dm = oracle.nw_3d(s1, s2, s3, c3)
+                assert costs[k] == want, (tcm, shapes[k])
+                bt = oracle.backtrack_3d(s1, s2, s3, dm)
+                assert [L(x) for x in outs[k][:3]] == [L(x) for x in bt], (tcm, shapes[k])
+        for s1, s2, s3 in trip[-3:]:
+            want, dm = oracle.nw_3d(s1, s2, s3, c3)
+            (r1, r2, r3), cost = al.align_3(s1, s2, s3)
+            assert cost == want
+            assert [L(r1), L(r2), L(r3)] == [L(x) for x in oracle.backtrack_3d(s1, s2, s3, dm)]
+    finally:
+        ctx.close()
