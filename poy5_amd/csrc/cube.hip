@@ -25,8 +25,9 @@
 // its 16 cells into a map y -> min(a, y + b), a Kogge-Stone scan over the
 // wavefront composes the maps, and a second pass applies the incoming value.
 //
-// Directions: one nibble per cell, 512 bytes per row segment, nibble order =
-// cell order.  The codes number the moves in the order they are tried
+// Directions: one nibble per cell, nibble order = cell order; a row of one
+// segment keeps the lane blocks below L3 (8 bytes each), a longer row 512
+// bytes per segment.  The codes number the moves in the order they are tried
 // (1 P3, 2 P1, 3 P2, 4 S3, 5 S1, 6 S2, 7 SS), so that among equal values the
 // first move tried has the smallest code.
 #pragma once
@@ -71,6 +72,7 @@ struct CubeFillArgs {
   long long ring_stride;   // ints per wavefront
   int* cost;               // per triple
   int* counter;
+  int seg_lo = 1, seg_hi = 1 << 30;  // the launch takes triples of seg_lo..seg_hi segments
 };
 
 __device__ __forceinline__ int cube_c3(const Cm3Dev* c, int a, int b, int x) {
@@ -292,7 +294,8 @@ __device__ int cube_fast(const CubeFillArgs& A, const uint8_t* s1, const uint8_t
       cube_row(kind, k0, cc, GG, U, P, D, 0, 0, 0, kCubeInf, s1g0, g0, s120, s_s1g, s_gs2, s_s12,
                X, lo, hi, y);
       store16(row_ptr(R), X);
-      if (kDm) reinterpret_cast<uint2*>(dm + (long long)R * 512)[lane] = make_uint2(lo, hi);
+      if (kDm && k0 < L3)
+        reinterpret_cast<uint2*>(dm + (long long)R * cube_dm_row_bytes(L3))[lane] = make_uint2(lo, hi);
       // register chains for the next row
       if (kind == kRowInner) {
 #pragma unroll
@@ -339,7 +342,7 @@ __global__ __launch_bounds__(64) void k_cube_fill(CubeFillArgs A) {
     const uint8_t* s3 = A.tt.blob + A.tt.off3[t];
     const int L1 = A.tt.len1[t], L2 = A.tt.len2[t], L3 = A.tt.len3[t];
     const int nseg = cube_nseg(L3);
-    if ((nseg == 1) != kOneSeg) continue;
+    if ((nseg == 1) != kOneSeg || nseg < A.seg_lo || nseg > A.seg_hi) continue;
     const int row_ints = nseg * kCubeSeg;
     const int NR = L1 + L2;
     int* ring = A.ring + (long long)blockIdx.x * A.ring_stride;
@@ -473,8 +476,7 @@ __global__ __launch_bounds__(64) void k_cube_backtrack(CubeBtArgs A) {
   const uint8_t* s3 = A.tt.blob + A.tt.off3[t];
   const int L1 = A.tt.len1[t], L2 = A.tt.len2[t], L3 = A.tt.len3[t];
   const int cap = L1 + L2 + L3;
-  const int nseg = cube_nseg(L3);
-  const long long row_bytes = (long long)nseg * 512;
+  const long long row_bytes = cube_dm_row_bytes(L3);
   const uint8_t* dm = A.dm + A.tt.dm_off[t];
   uint8_t* r1 = A.out + A.tt.out_off[t];
   uint8_t* r2 = r1 + cap;

@@ -1417,10 +1417,11 @@ int run_cube_one(poyhip_ctx* c, const uint8_t* s1, int l1, const uint8_t* s2, in
   A.cost = dcost;
   A.counter = &reinterpret_cast<OneDesc3*>(c->stage.p)->counter;
   const int64_t lsum = (int64_t)l1 + l2 + l3;
-  if (cube_nseg(l3) == 1 && cube_mw_ok(c, lsum) && cube_mw_u16(lsum, c->cm3_max))
+  const bool mw = cube_nseg(l3) <= kCubeMwMaxSeg && cube_mw_ok(c, lsum);
+  if (mw && cube_mw_u16(lsum, c->cm3_max))
     hipLaunchKernelGGL((k_cube_mw<true, true, kCubeMwWaves>), dim3(1), dim3(64 * kCubeMwWaves), 0,
                        c->stream, A);
-  else if (cube_nseg(l3) == 1 && cube_mw_ok(c, lsum))
+  else if (mw)
     hipLaunchKernelGGL((k_cube_mw<true, false, kCubeMwWaves>), dim3(1), dim3(64 * kCubeMwWaves),
                        0, c->stream, A);
   else if (cube_nseg(l3) == 1)
@@ -1596,7 +1597,7 @@ struct poyhip_batch3 {
   int64_t dm_cap = 0;
   long long ring_stride = 0;
   int grid = 0;
-  int64_t len_sum_1seg = 0;  // largest L1 + L2 + L3 among one-segment triples
+  int64_t len_sum_mw = 0;  // largest L1 + L2 + L3 among triples of <= kCubeMwMaxSeg segments
   size_t out_bytes = 0;
   std::vector<hipEvent_t> ev;
   float fill_ms = -1.f, total_ms = -1.f;
@@ -1642,9 +1643,11 @@ int poyhip_batch3_create(poyhip_ctx* c, int op, int n, const int* len1, const in
       delete b;
       return set_err(c, POYHIP_EINVAL, "triple %d has an empty sequence", k);
     }
-    work[k] = (double)len1[k] * len2[k] * len3[k];
-    if (cube_nseg(len3[k]) == 1)
-      b->len_sum_1seg = std::max<int64_t>(b->len_sum_1seg, (int64_t)len1[k] + len2[k] + len3[k]);
+    // fill time follows the row count (a row is one wavefront's pass over
+    // its segments, whatever L3)
+    work[k] = (double)len1[k] * len2[k] * cube_nseg(len3[k]);
+    if (cube_nseg(len3[k]) <= kCubeMwMaxSeg)
+      b->len_sum_mw = std::max<int64_t>(b->len_sum_mw, (int64_t)len1[k] + len2[k] + len3[k]);
     b->ring_stride = std::max<long long>(b->ring_stride, cube_ring_ints(len1[k], len2[k], len3[k]));
     b->ring_stride = std::max<long long>(
         b->ring_stride, cube_mw_ring_ints(len1[k], len2[k], len3[k], kCubeMwWaves));
@@ -1673,9 +1676,16 @@ int poyhip_batch3_create(poyhip_ctx* c, int op, int n, const int* len1, const in
   }
   b->grid = (int)grid;
   if (op == POYHIP_OP_ALIGN_3D) {
-    if (dm_budget == 0) dm_budget = std::min(fr / 2, (size_t)160 << 30);
+    // default: three quarters of the memory the rings leave free
+    const size_t rings = sizeof(int) * (size_t)b->ring_stride * b->grid;
+    if (dm_budget == 0) dm_budget = fr > rings ? (fr - rings) / 4 * 3 : 0;
+    // A chunk takes no more work than its resident workgroups finish in the
+    // time of its first (largest) triple, so that its work-sorted triples
+    // run side by side without a tail.
+    const double slots = c->cube_mw ? c->num_cus * 2 : b->grid;
     int64_t cur = 0;
     int begin = 0;
+    double wsum = 0.0;
     for (int q = 0; q < n; q++) {
       const int k = b->order[q];
       const int64_t sz = (cube_dm_bytes(len1[k], len2[k], len3[k]) + 255) & ~(int64_t)255;
@@ -1683,13 +1693,16 @@ int poyhip_batch3_create(poyhip_ctx* c, int op, int n, const int* len1, const in
         delete b;
         return set_err(c, POYHIP_ECAP, "triple %d needs %lld direction bytes", k, (long long)sz);
       }
-      if (cur > 0 && cur + sz > (int64_t)dm_budget) {
+      if (cur > 0 &&
+          (cur + sz > (int64_t)dm_budget || wsum + work[k] > slots * work[b->order[begin]])) {
+        wsum = 0.0;
         b->chunks.push_back({begin, q - begin, cur});
         begin = q;
         cur = 0;
       }
       b->dm_off[k] = cur;
       cur += sz;
+      wsum += work[k];
     }
     b->chunks.push_back({begin, n - begin, cur});
     for (const auto& ch : b->chunks) b->dm_cap = std::max(b->dm_cap, ch.dm_bytes);
@@ -1795,15 +1808,19 @@ int poyhip_batch3_run(poyhip_batch3* b, const uint8_t* d_blob, const int64_t* d_
     A.counter = b->d_counters + k;
     const int g = std::min(b->grid, ch.count);
     HIPCHK(c, hipEventRecord(b->ev[2 * k], st));
-    // one launch per row shape (counters 2k and 2k+1): one-segment triples
-    // on workgroups of kCubeMwWaves wavefronts (two per CU), the longer ones
-    // on single wavefronts
+    // two launches (counters 2k and 2k+1): triples of up to kCubeMwMaxSeg
+    // segments on workgroups of kCubeMwWaves wavefronts (two per CU), the
+    // longer ones on single wavefronts; when the keys of the workgroup fill
+    // could wrap, one-segment and longer triples on single wavefronts
     A.counter = b->d_counters + 2 * k;
     const bool dm = b->op == POYHIP_OP_ALIGN_3D;
-    if (cube_mw_ok(c, b->len_sum_1seg)) {
+    const bool mw = cube_mw_ok(c, b->len_sum_mw);
+    if (mw) {
       const int gm = std::min(g, c->num_cus * 2);
       const dim3 blk(64 * kCubeMwWaves);
-      if (cube_mw_u16(b->len_sum_1seg, c->cm3_max)) {
+      A.seg_lo = 1;
+      A.seg_hi = kCubeMwMaxSeg;
+      if (cube_mw_u16(b->len_sum_mw, c->cm3_max)) {
         if (dm)
           hipLaunchKernelGGL((k_cube_mw<true, true, kCubeMwWaves>), dim3(gm), blk, 0, st, A);
         else
@@ -1819,6 +1836,8 @@ int poyhip_batch3_run(poyhip_batch3* b, const uint8_t* d_blob, const int64_t* d_
       hipLaunchKernelGGL((k_cube_fill<false, true>), dim3(g), dim3(64), 0, st, A);
     }
     A.counter = b->d_counters + 2 * k + 1;
+    A.seg_lo = mw ? kCubeMwMaxSeg + 1 : 2;
+    A.seg_hi = 1 << 30;
     if (b->op == POYHIP_OP_ALIGN_3D)
       hipLaunchKernelGGL((k_cube_fill<true, false>), dim3(g), dim3(64), 0, st, A);
     else

@@ -48,9 +48,13 @@ POYHIP_HD inline int64_t cube_row_ints(int L3) { return (int64_t)cube_nseg(L3) *
 POYHIP_HD inline int64_t cube_ring_ints(int L1, int L2, int L3) {
   return (int64_t)(L1 + L2) * cube_row_ints(L3);
 }
-// one nibble per cell, 512 bytes per row segment
+// one nibble per cell: a one-segment row keeps the 8-byte lane blocks that
+// carry cells below L3, a longer row 512 bytes per segment
+POYHIP_HD inline int64_t cube_dm_row_bytes(int L3) {
+  return cube_nseg(L3) == 1 ? 8 * (int64_t)((L3 + 15) / 16) : (int64_t)cube_nseg(L3) * 512;
+}
 POYHIP_HD inline int64_t cube_dm_bytes(int L1, int L2, int L3) {
-  return (int64_t)L1 * L2 * cube_nseg(L3) * 512;
+  return (int64_t)L1 * L2 * cube_dm_row_bytes(L3);
 }
 // outputs: r1, r2, r3, median_3, each L1+L2+L3 bytes
 POYHIP_HD inline int64_t cube_out_cap(int L1, int L2, int L3) { return (int64_t)L1 + L2 + L3; }

@@ -131,6 +131,9 @@ def test_batch_3d_workgroup_fill_equals_wavefront_fill(env):
     for _ in range(24):
         n = rng.integers(1, 90, 3)
         trip.append(tuple(rseq(rng, int(x) - 1) for x in n))
+    # rows of two and three segments
+    for n in [(6, 9, 1030), (12, 5, 2100), (3, 17, 3072), (9, 9, 1025)]:
+        trip.append(tuple(rseq(rng, x - 1) for x in n))
     costs, outs = batch.run_triples(ctx, "align_3d", trip)
     os.environ["POYHIP_CUBE_FILL"] = "wave"
     try:
@@ -149,7 +152,7 @@ def test_batch_3d_workgroup_fill_equals_wavefront_fill(env):
     assert np.array_equal(wc2, costs)
     for k in range(len(trip)):
         assert [L(x) for x in outs[k]] == [L(x) for x in wouts[k]], k
-    for k in range(2):
+    for k in [0, 1, len(trip) - 4, len(trip) - 3]:
         s1, s2, s3 = trip[k]
         want, dm = oracle.nw_3d(s1, s2, s3, c3)
         assert costs[k] == want
