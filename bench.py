@@ -308,11 +308,16 @@ def serving(n_req):
         t0 = time.perf_counter()
         mid = spr.downpass(srv, parent, children, leaves)
         down_s = time.perf_counter() - t0
-        res = spr.run_loads(srv, children, mid, n_req=n_req)
+        t0 = time.perf_counter()
+        emid = spr.edge_medians(srv, parent, mid)
+        edge_s = time.perf_counter() - t0
+        res = spr.run_loads(srv, parent, children, mid, emid, n_req=n_req)
         res["workload"] = ("128-taxon Yule tree, root 1500, JC + 2% indels; down pass "
-                           "(127 align_2 + ancestor, medians kept in the device pool), then "
-                           "cost_2(clade median, node median) rejoin requests")
+                           "(127 align_2 + ancestor, medians kept in the device pool), edge "
+                           "medians (254 align_2 + ancestor), then cost_2(clade median, edge "
+                           "median) rejoin requests with deltaw = max(|la-lb|, 8)")
         res["downpass_ms"] = round(down_s * 1e3, 3)
+        res["edge_medians_ms"] = round(edge_s * 1e3, 3)
         res["server"] = srv.stats()
         return res
     finally:

@@ -8,9 +8,13 @@ client mirrors what POY's DO search asks of ``Sequence.Align``:
 * the down pass computes every internal node's median as the ancestor of
   the aligned child medians (``align_2`` + ``ancestor_2``,
   src/seqCS.ml:985-1084), keeping each median in the server's device pool;
-* an SPR round clips a subtree and prices its rejoin at every node of the
-  remaining tree with ``cost_2(clade median, node median)``
-  (src/seqCS.ml:701-774), all on pool-resident medians.
+* every edge (v, parent v) carries its own median, the ancestor of the
+  two node medians (the edge data ``cost_fn`` prices a rejoin against,
+  src/allDirChar.ml:2132-2178);
+* an SPR round clips a subtree and prices its rejoin at every edge of the
+  remaining tree with ``cost_2(clade median, edge median)`` and
+  ``DOS.distance``'s band ``deltaw = max(|la - lb|, 8)``
+  (src/seqCS.ml:701-744), all on pool-resident medians.
 
 Rejoin requests arrive as Poisson traffic from a native load generator
 (``poyhip_server_replay``), at fractions of the measured batched capacity.
@@ -114,6 +118,30 @@ def downpass(server, parent, children, leaves):
     return mid
 
 
+def edge_medians(server, parent, mid):
+    """Median of every edge (v, parent v), v not the root: {v: pool id}.
+    An edge's data in the all-directions tree is the median of its two end
+    nodes, computed like a down-pass median (align_2 + ancestor)."""
+    emid, reqs = {}, []
+    for v in mid:
+        if parent[v] < 0:
+            continue
+        r, m = server.submit("align_2d", mid[v], mid[parent[v]], keep_median=True)
+        emid[v] = m
+        reqs.append(r)
+    for r in reqs:
+        server.wait(r)
+        server.result(r)
+    return emid
+
+
+def dos_deltaw(la: int, lb: int) -> int:
+    """``DOS.distance``'s band for ``cost_2`` (src/seqCS.ml:738-744): the
+    length difference, at least 8."""
+    d = abs(int(la) - int(lb))
+    return d if d > 8 else 8
+
+
 def subtree(children, v):
     out, st = [], [v]
     while st:
@@ -123,22 +151,28 @@ def subtree(children, v):
     return set(out)
 
 
-def rejoin_requests(children, mid, n_req, rng):
-    """(a ids, b ids): cost_2(clade median, node median) for the nodes
-    outside the clipped subtree, clip after clip, until n_req requests."""
-    nodes = list(mid)
-    a, b = [], []
+def rejoin_requests(parent, children, mid, emid, n_req, rng):
+    """(a ids, b ids, clipped nodes, edge nodes): cost_2(clade median, edge
+    median) for the edges of the tree left after a clip, clip after clip,
+    until n_req requests.  The clipped subtree's own edges, and the edge
+    above the clip's parent (which the clip merges with the sibling's edge,
+    still priced through the sibling's edge median), are not junctions."""
+    nodes = [v for v in mid if parent[v] >= 0]
+    a, b, cv, ev = [], [], [], []
     while len(a) < n_req:
-        c = int(rng.choice(nodes[1:]))  # never clip the root
+        c = int(rng.choice(nodes))
         inside = subtree(children, c)
         for v in nodes:
-            if v in inside:
+            if v in inside or v == parent[c]:
                 continue
             a.append(mid[c])
-            b.append(mid[v])
+            b.append(emid[v])
+            cv.append(c)
+            ev.append(v)
             if len(a) == n_req:
                 break
-    return np.array(a, np.int64), np.array(b, np.int64)
+    return (np.array(a, np.int64), np.array(b, np.int64), np.array(cv, np.int64),
+            np.array(ev, np.int64))
 
 
 def poisson_arrivals(n, rate_per_s, rng):
@@ -147,26 +181,26 @@ def poisson_arrivals(n, rate_per_s, rng):
     return np.cumsum(gaps) - gaps[0]
 
 
-def run_loads(server, children, mid, n_req=4000, loads=(0.25, 0.5, 0.9), seed=7):
+def rejoin_deltaw(server, a, b):
+    """Per-request ``DOS.distance`` band of the pool medians a[k], b[k];
+    also returns the pool lengths read."""
+    lens = {}
+    for x in np.concatenate([a, b]):
+        if int(x) not in lens:
+            lens[int(x)] = len(server.get(int(x)))
+    dw = np.array([dos_deltaw(lens[int(x)], lens[int(y)]) for x, y in zip(a, b)], np.int32)
+    return dw, lens
+
+
+def run_loads(server, parent, children, mid, emid, n_req=4000, loads=(0.25, 0.5, 0.9), seed=7):
     """Capacity (all requests at once) and Poisson runs at fractions of it.
     Returns a dict of capacity, and per load p50/p99 latency and GCUPS."""
     rng = np.random.default_rng(seed)
-    a, b = rejoin_requests(children, mid, n_req, rng)
-    lens = {}
-
-    def cells(av, bv):
-        tot = 0.0
-        for x, y in zip(av, bv):
-            if x not in lens:
-                lens[x] = len(server.get(int(x)))
-            if y not in lens:
-                lens[y] = len(server.get(int(y)))
-            tot += float(lens[x]) * lens[y]
-        return tot
-
-    cells_all = cells(a, b)
+    a, b, _, _ = rejoin_requests(parent, children, mid, emid, n_req, rng)
+    dw, lens = rejoin_deltaw(server, a, b)
+    cells_all = float(sum(float(lens[int(x)]) * lens[int(y)] for x, y in zip(a, b)))
     ops = ["cost_2d"] * n_req
-    costs0, lat0, _, wall0 = server.replay(np.zeros(n_req), ops, a, b)
+    costs0, lat0, _, wall0 = server.replay(np.zeros(n_req), ops, a, b, deltaw=dw)
     cap_rps = n_req / (wall0 / 1e6)
     res = {"requests": n_req, "cells_full_per_request": cells_all / n_req,
            "capacity_rps": cap_rps, "capacity_gcups": cells_all / (wall0 / 1e6) / 1e9,
@@ -174,7 +208,7 @@ def run_loads(server, children, mid, n_req=4000, loads=(0.25, 0.5, 0.9), seed=7)
            "saturated_p99_ms": float(np.percentile(lat0, 99)) / 1e3, "loads": []}
     for rho in loads:
         arr = poisson_arrivals(n_req, rho * cap_rps, rng)
-        costs, lat, _, wall = server.replay(arr, ops, a, b)
+        costs, lat, _, wall = server.replay(arr, ops, a, b, deltaw=dw)
         res["loads"].append({
             "load": rho, "offered_rps": rho * cap_rps, "achieved_rps": n_req / (wall / 1e6),
             "gcups": cells_all / (wall / 1e6) / 1e9,

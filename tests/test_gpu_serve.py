@@ -29,27 +29,51 @@ def oracle_medians(children, leaves, ocm):
     return med
 
 
-def test_downpass_and_rejoins_match_oracle(ctx):
-    print("test_downpass_and_rejoins_match_oracle", flush=True)
+def check_spr_workload(ctx, n_taxa, root_len, seed, n_req, slot_dm_bytes):
+    """Down pass, edge medians and rejoin costs of a simulated tree on the
+    server, each against the oracle: node medians (align_2 + ancestor_2),
+    edge medians (the same on the edge's two node medians) and
+    cost_2(clade median, edge median) with DOS.distance's deltaw."""
     ocm = oracle.cm_dna(1, 2)
-    parent, children, leaves = spr.simulate(n_taxa=16, root_len=300, seed=11)
-    srv = Server(ctx, n_slots=3, max_batch_pairs=256, max_len=2048, slot_dm_bytes=64 << 20,
-                 pool_bytes=16 << 20)
+    parent, children, leaves = spr.simulate(n_taxa=n_taxa, root_len=root_len, seed=seed)
+    srv = Server(ctx, n_slots=3, max_batch_pairs=2048, max_len=4096,
+                 slot_dm_bytes=slot_dm_bytes, pool_bytes=64 << 20)
     try:
         mid = spr.downpass(srv, parent, children, leaves)
         want = oracle_medians(children, leaves, ocm)
         for v in mid:
             assert np.array_equal(srv.get(mid[v]), want[v]), v
+        emid = spr.edge_medians(srv, parent, mid)
+        ewant = {}
+        for v in emid:
+            r1, r2, _, _ = oracle.align_2(want[v], want[parent[v]], ocm)
+            ewant[v] = oracle.ancestor_2(r1, r2, ocm)
+            assert np.array_equal(srv.get(emid[v]), ewant[v]), ("edge", v)
         rng = np.random.default_rng(3)
-        a, b = spr.rejoin_requests(children, mid, 300, rng)
-        inv = {m: v for v, m in mid.items()}
-        costs, lat, _, wall = srv.replay(np.zeros(len(a)), ["cost_2d"] * len(a), a, b)
+        a, b, cv, ev = spr.rejoin_requests(parent, children, mid, emid, n_req, rng)
+        dw, _ = spr.rejoin_deltaw(srv, a, b)
+        costs, lat, _, wall = srv.replay(np.zeros(len(a)), ["cost_2d"] * len(a), a, b, deltaw=dw)
         for k in range(len(a)):
-            oc = oracle.cost_2(want[inv[a[k]]], want[inv[b[k]]], ocm)[0]
+            x, y = want[int(cv[k])], ewant[int(ev[k])]
+            assert dw[k] == max(abs(len(x) - len(y)), 8)
+            oc = oracle.cost_2(x, y, ocm, deltaw=int(dw[k]))[0]
             assert costs[k] == oc, k
         assert np.all(lat > 0)
     finally:
         srv.close()
+
+
+def test_downpass_and_rejoins_match_oracle(ctx):
+    print("test_downpass_and_rejoins_match_oracle", flush=True)
+    check_spr_workload(ctx, 16, 300, 11, 300, 64 << 20)
+
+
+def test_spr_config3_full_size(ctx):
+    """The serving workload at its configuration: the seeded 128-taxon tree
+    with root length 1,500 (spr.simulate defaults), all 127 down-pass
+    medians, all 254 edge medians and 1,200 rejoin costs."""
+    print("test_spr_config3_full_size", flush=True)
+    check_spr_workload(ctx, 128, 1500, 0x504F59003, 1200, 1 << 30)
 
 
 def test_alone_equals_mix(ctx):
