@@ -52,6 +52,8 @@ def parse():
     ap.add_argument("--c1-pairs", type=int, default=1024)
     ap.add_argument("--c4-pairs", type=int, default=2048)
     ap.add_argument("--c5-triples", type=int, default=16384)
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default tuning knobs instead of autotuning config 2")
     ap.add_argument("--graph", type=int, default=0,
                     help="replay each step as one HIP graph (no per-kernel events then)")
     ap.add_argument("--dm-budget-gb", type=float, default=0.0)
@@ -369,6 +371,14 @@ def run_bench(args, pool, procs):
 
     ctx = Context(0)
     ctx.set_cm(cost_matrix.of_transformations_and_gaps(1, 1, 3))
+    tune = None
+    if not args.no_tune:
+        # outside the timed region: knobs per length bucket on samples of
+        # the batch, the best predicted single setting applied
+        from poy5_amd import autotune
+        print("[bench] autotuning", file=sys.stderr, flush=True)
+        tune = autotune.tune_pairs(ctx, "align_affine", blob, off[0::2], off[1::2], len1, len2)
+        print(f"[bench] autotune choice {tune['choice']}", file=sys.stderr, flush=True)
     budget = int(args.dm_budget_gb * (1 << 30))
     b = Batch(ctx, "align_affine", len1, len2, dm_budget=budget)
     dev = torch.device("cuda:0")
@@ -449,7 +459,8 @@ def run_bench(args, pool, procs):
                    "cells_computed": cells_computed, "lane_steps": lane_slots,
                    "lane_occupancy": round(cells_computed / max(lane_slots, 1.0), 4),
                    "mean_rounds": float(rounds.mean()),
-                   "graph": bool(args.graph), "dm_budget_gb": args.dm_budget_gb},
+                   "graph": bool(args.graph), "dm_budget_gb": args.dm_budget_gb,
+                   "autotune": tune},
         "roofline": roofline,
     }
     if roofline is None:

@@ -216,6 +216,30 @@ int poyhip_batch_stats(poyhip_batch *b, int64_t *cells, int64_t *slots, int *rou
 /* Captures the batch's launches into a HIP graph and replays it on later
  * runs with the same device pointers. */
 int poyhip_batch_use_graph(poyhip_batch *b, int enable);
+
+/* Tuning knobs of a context (no reference counterpart: the settings the
+ * autotuner, poy5_amd/autotune.py, chooses per length bucket).  A value
+ * applies to batches created and runs started after it is set.
+ *   WAVES_PER_CU      resident wavefronts per CU of a pair batch's
+ *                     persistent fill, 1..32 (default 16)
+ *   AFFINE_ROTATING   1: gap-free affine pairs use the rotating stripe
+ *                     schedule within its limits (default); 0: the plain
+ *                     stripe loop
+ *   LINEAR_WORKGROUP  1: the linear fill runs a workgroup of wavefronts per
+ *                     pair when every cost entry fits 16 bits (default);
+ *                     0: one wavefront per pair
+ *   CUBE_WORKGROUP    1: the 3D fill runs a workgroup per triple when its
+ *                     keys fit (default); 0: one wavefront per triple
+ * Results do not depend on the knobs.  get returns the value or
+ * POYHIP_EINVAL. */
+enum {
+  POYHIP_TUNE_WAVES_PER_CU = 1,
+  POYHIP_TUNE_AFFINE_ROTATING = 2,
+  POYHIP_TUNE_LINEAR_WORKGROUP = 3,
+  POYHIP_TUNE_CUBE_WORKGROUP = 4
+};
+int poyhip_set_tuning(poyhip_ctx *c, int knob, int value);
+int poyhip_get_tuning(poyhip_ctx *c, int knob);
 /* Timing of the last non-graph run with sync set, from HIP events recorded
  * on the context stream: last_fill_ms is the summed duration of the fill
  * stage over all chunks (for ALIGN_AFFINE a chunk's fill stage is the

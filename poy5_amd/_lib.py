@@ -125,6 +125,8 @@ SIGNATURES = {
     "poyhip_batch_run": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_int]),
     "poyhip_batch_stats": (C.c_int, [_vp, _i64p, _i64p, _ip, _ip]),
     "poyhip_batch_use_graph": (C.c_int, [_vp, C.c_int]),
+    "poyhip_set_tuning": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "poyhip_get_tuning": (C.c_int, [_vp, C.c_int]),
     "poyhip_batch_last_fill_ms": (C.c_float, [_vp]),
     "poyhip_batch_last_total_ms": (C.c_float, [_vp]),
     "poyhip_batch_chunks": (C.c_int, [_vp]),

@@ -34,6 +34,7 @@ struct AffFillArgs {
   // kernel's wavefronts run each pair's backtrace right after its fill
   uint8_t* out = nullptr;
   int* outlen = nullptr;
+  int rotating = 1;  // 0: gap-free pairs also run the plain stripe loop
 };
 
 __device__ __forceinline__ void aff_backtrace_pair(PairTable pt, const uint8_t* dm_all,
@@ -777,8 +778,9 @@ __global__ __launch_bounds__(64, 4) void k_affine_fill(AffFillArgs A) {
       int rot_total = -1;
       if constexpr (kFast && kDm) {
         int lag = 0;
-        rot_total =
-            rot_plan(s_rot, leni, lenj, D, newk, nstripes, aff_dm_bytes(La, Lb) >> 5, lag);
+        if (A.rotating)
+          rot_total =
+              rot_plan(s_rot, leni, lenj, D, newk, nstripes, aff_dm_bytes(La, Lb) >> 5, lag);
         if (rot_total > 0) {
           // the global FIFO (Lb + 64 entries of 4 ints) reuses the stripe
           // loop's row buffers (16 * Lb + 16 ints; the spill needs lenj > 400)

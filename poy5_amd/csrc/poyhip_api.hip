@@ -75,6 +75,13 @@ struct poyhip_ctx {
   // one-segment triples (L3 <= 1024) fill with one workgroup of wavefronts
   // per triple; POYHIP_CUBE_FILL=wave forces the wavefront-per-triple kernel
   bool cube_mw = true;
+  // tuning knobs (poyhip_set_tuning): resident wavefronts per CU of the
+  // pair batches, the rotating schedule of gap-free affine pairs, and the
+  // workgroup linear fill (used when the costs also fit, lin_fits)
+  int waves_per_cu = 16;
+  bool aff_rot = true;
+  bool lin_wg = true;
+  bool lin_fits = false;
   // largest entry of the 3D cost table, -1 when an entry is negative (the
   // workgroup fill carries 8*value + code keys and needs values below 2^27)
   int cm3_max = -1;
@@ -351,6 +358,8 @@ int poyhip_create(int device, poyhip_ctx** out) {
   if (c->num_cus <= 0) c->num_cus = 256;
   const char* e = getenv("POYHIP_CUBE_FILL");
   c->cube_mw = !(e && strcmp(e, "wave") == 0);
+  e = getenv("POYHIP_LINEAR_FILL");
+  c->lin_wg = !(e && strcmp(e, "wave") == 0);
   *out = c;
   return POYHIP_OK;
 }
@@ -416,8 +425,8 @@ int poyhip_set_cm(poyhip_ctx* c, const poyhip_cm* cm) {
   c->last2d = false;
   bool fits = true;
   for (int k = 0; k < 1024; k++) fits &= cm->cost[k] >= 0 && cm->cost[k] < 32768;
-  const char* e = getenv("POYHIP_LINEAR_FILL");
-  c->lin_mw = fits && !(e && strcmp(e, "wave") == 0);
+  c->lin_fits = fits;
+  c->lin_mw = fits && c->lin_wg;
   return POYHIP_OK;
 }
 
@@ -543,6 +552,7 @@ int poyhip_align_affine_3(poyhip_ctx* c, const uint8_t* si, int li, const uint8_
   int* doutlen = reinterpret_cast<int*>(static_cast<char*>(c->outblk.p) + sizeof(PairResult));
   uint8_t* dout = static_cast<uint8_t*>(c->outblk.p) + kResBytes;
   AffFillArgs A;
+  A.rotating = c->aff_rot ? 1 : 0;
   A.pt = t;
   A.cm = c->d_cm;
   A.dm = static_cast<uint8_t*>(c->dm.p);
@@ -629,6 +639,7 @@ int poyhip_align_affine_3_noalloc(poyhip_ctx* c, const uint8_t* si, int li, cons
   int* doutlen = reinterpret_cast<int*>(S + L.out + sizeof(PairResult));
   uint8_t* dout = S + L.out + kResBytes;
   AffFillArgs A;
+  A.rotating = c->aff_rot ? 1 : 0;
   A.pt = t;
   A.cm = c->d_cm;
   A.dm = S + L.dm;
@@ -918,6 +929,7 @@ int batch_enqueue(poyhip_batch* b, const uint8_t* blob, const int64_t* off1, con
         } else {
           // gap-free pairs on the fast kernel, the others on the general one
           AffFillArgs A;
+          A.rotating = c->aff_rot ? 1 : 0;
           A.pt = t;
           A.cm = c->d_cm;
           A.dm = b->d_dm;
@@ -1071,7 +1083,7 @@ int poyhip_batch_create(poyhip_ctx* c, int op, int n, const int* len1, const int
   }
   for (const auto& ch : b->chunks) b->dm_cap = std::max(b->dm_cap, ch.dm_bytes);
   // persistent grid: waves resident across the device
-  b->grid = std::min(n, c->num_cus * 16);
+  b->grid = std::min(n, c->num_cus * c->waves_per_cu);
   if (b->grid < 1) b->grid = 1;
   b->ws_stride = (ws_ints(op, maxlb) + 63) & ~63LL;
   // device allocations
@@ -1153,6 +1165,39 @@ int poyhip_batch_out_cap(const poyhip_batch* b, int k) {
   if (b->op == POYHIP_OP_ALIGN_AFFINE) return b->la[k] + b->lb[k] + 2;
   if (b->op == POYHIP_OP_ALIGN_2D) return (int)lin_out_cap(b->la[k], b->lb[k]);
   return 0;
+}
+
+int poyhip_set_tuning(poyhip_ctx* c, int knob, int value) {
+  if (!c) return POYHIP_EINVAL;
+  switch (knob) {
+    case POYHIP_TUNE_WAVES_PER_CU:
+      if (value < 1 || value > 32) return set_err(c, POYHIP_EINVAL, "waves per CU %d", value);
+      c->waves_per_cu = value;
+      return POYHIP_OK;
+    case POYHIP_TUNE_AFFINE_ROTATING:
+      c->aff_rot = value != 0;
+      return POYHIP_OK;
+    case POYHIP_TUNE_LINEAR_WORKGROUP:
+      c->lin_wg = value != 0;
+      c->lin_mw = c->lin_fits && c->lin_wg;
+      return POYHIP_OK;
+    case POYHIP_TUNE_CUBE_WORKGROUP:
+      c->cube_mw = value != 0;
+      return POYHIP_OK;
+    default:
+      return set_err(c, POYHIP_EINVAL, "unknown tuning knob %d", knob);
+  }
+}
+
+int poyhip_get_tuning(poyhip_ctx* c, int knob) {
+  if (!c) return POYHIP_EINVAL;
+  switch (knob) {
+    case POYHIP_TUNE_WAVES_PER_CU: return c->waves_per_cu;
+    case POYHIP_TUNE_AFFINE_ROTATING: return c->aff_rot ? 1 : 0;
+    case POYHIP_TUNE_LINEAR_WORKGROUP: return c->lin_wg ? 1 : 0;
+    case POYHIP_TUNE_CUBE_WORKGROUP: return c->cube_mw ? 1 : 0;
+    default: return POYHIP_EINVAL;
+  }
 }
 
 int poyhip_batch_use_graph(poyhip_batch* b, int enable) {

@@ -302,6 +302,7 @@ int srv_launch(poyhip_server* s, SrvSlot& sl) {
                        dt.dwin);
   if (op == POYHIP_OP_ALIGN_AFFINE) {
     AffFillArgs A;
+    A.rotating = c->aff_rot ? 1 : 0;
     A.pt = pt;
     A.cm = c->d_cm;
     A.dm = sl.d_dm;

@@ -1,0 +1,83 @@
+"""Tuning knobs and the autotuner: every setting gives the same costs and
+output sequences; the autotuner picks one of its candidates and leaves it on
+the context."""
+import numpy as np
+import pytest
+
+from poy5_amd import autotune, cost_matrix, workloads
+from poy5_amd._lib import PoyhipError
+from poy5_amd.batch import pack_pairs, run_pairs, run_triples
+from poy5_amd.sequence import Context
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = Context(0)
+    yield c
+    c.close()
+
+
+def same_outputs(x, y):
+    return all(np.array_equal(p, q) for a, b in zip(x, y) for p, q in zip(a, b))
+
+
+@pytest.mark.parametrize("op,model", [("align_affine", (1, 1, 3)), ("align_2d", (1, 2, None))])
+def test_results_do_not_depend_on_knobs(ctx, op, model):
+    print(f"test_results_do_not_depend_on_knobs {op}", flush=True)
+    ctx.set_cm(cost_matrix.of_transformations_and_gaps(*model))
+    pairs = workloads.pairs_list(2, 128)
+    pairs = [p for p in pairs if len(p[0]) * len(p[1]) < 3_000_000][:64]
+    default = autotune.get_knobs(ctx)
+    try:
+        base_c, base_o = run_pairs(ctx, op, pairs)
+        for cand in autotune.candidates(op):
+            autotune.set_knobs(ctx, cand)
+            c, o = run_pairs(ctx, op, pairs)
+            assert np.array_equal(c, base_c), cand
+            assert same_outputs(o, base_o), cand
+    finally:
+        autotune.set_knobs(ctx, default)
+
+
+def test_cube_knob(ctx):
+    print("test_cube_knob", flush=True)
+    two = cost_matrix.of_transformations_and_gaps(1, 1)
+    ctx.set_cm(two)
+    ctx.set_cm3(cost_matrix.three_d_of_two_dim_comb(two))
+    rng = np.random.default_rng(9)
+    trip = [tuple(np.concatenate([[16], rng.choice([1, 2, 4, 8], int(n))]).astype(np.uint8)
+                  for n in rng.integers(20, 120, 3)) for _ in range(12)]
+    c1, o1 = run_triples(ctx, "align_3d", trip)
+    autotune.set_knobs(ctx, {"cube_workgroup": 0})
+    try:
+        c0, o0 = run_triples(ctx, "align_3d", trip)
+    finally:
+        autotune.set_knobs(ctx, {"cube_workgroup": 1})
+    assert np.array_equal(c0, c1) and same_outputs(o0, o1)
+
+
+def test_autotune_picks_a_candidate(ctx):
+    print("test_autotune_picks_a_candidate", flush=True)
+    ctx.set_cm(cost_matrix.of_transformations_and_gaps(1, 1, 3))
+    blob, off, ln = workloads.gen_pairs(2, 600)
+    l1, l2 = ln[0::2].astype(np.int32), ln[1::2].astype(np.int32)
+    default = autotune.get_knobs(ctx)
+    try:
+        res = autotune.tune_pairs(ctx, "align_affine", blob, off[0::2], off[1::2], l1, l2,
+                                  sample=48, repeats=1)
+        assert res["choice"] in autotune.candidates("align_affine")
+        got = autotune.get_knobs(ctx)
+        assert all(got[k] == v for k, v in res["choice"].items())
+        assert len(res["buckets"]) >= 2
+    finally:
+        autotune.set_knobs(ctx, default)
+
+
+def test_set_tuning_rejects_bad_values(ctx):
+    with pytest.raises(PoyhipError):
+        autotune.set_knobs(ctx, {"waves_per_cu": 0})
+    with pytest.raises(PoyhipError):
+        ctx.chk(ctx._lib.poyhip_set_tuning(ctx.h, 99, 1))
+    assert ctx._lib.poyhip_get_tuning(ctx.h, 99) < 0
