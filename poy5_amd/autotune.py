@@ -12,11 +12,15 @@ bucket with the batch kernels themselves (fill stage, HIP events, best of a
 few runs) and keeps the fastest per bucket.  One persistent launch serves
 the whole batch, so the setting applied is the one with the smallest
 predicted total: each bucket's sample time scaled by the bucket's share of
-the cells.
+the cells.  A sample smaller than the persistent grid would run every
+pair at once whatever the occupancy knob, so a bucket's sample is large
+(8,192 pairs or the whole bucket), and the default setting is kept unless
+another is predicted at least 2% faster.
 """
 from __future__ import annotations
 
 import itertools
+import time
 
 import numpy as np
 import torch
@@ -49,12 +53,11 @@ def candidates(op: str) -> list[dict]:
             for w, g in itertools.product((8, 16), (1, 0))]
 
 
-def _time_fill(ctx, op, blob, off1, off2, len1, len2, idx, repeats, device):
+def _time_fill(ctx, op, d_blob, off1, off2, len1, len2, idx, repeats, device):
     """Best fill time (ms) of the pairs idx as one batch."""
     sub_len1, sub_len2 = len1[idx], len2[idx]
     b = Batch(ctx, op, sub_len1, sub_len2)
     try:
-        d_blob = torch.from_numpy(blob).to(device)
         d1 = torch.from_numpy(np.ascontiguousarray(off1[idx])).to(device)
         d2 = torch.from_numpy(np.ascontiguousarray(off2[idx])).to(device)
         cost, out, outlen = b.alloc_outputs(device)
@@ -67,11 +70,12 @@ def _time_fill(ctx, op, blob, off1, off2, len1, len2, idx, repeats, device):
         b.close()
 
 
-def tune_pairs(ctx: Context, op: str, blob, off1, off2, len1, len2, sample=192, repeats=2,
-               seed=1, device="cuda", apply=True) -> dict:
+def tune_pairs(ctx: Context, op: str, blob, off1, off2, len1, len2, sample=8192, repeats=2,
+               seed=1, device="cuda", apply=True, margin=0.02) -> dict:
     """Tunes the knobs for a pair batch (host arrays of the batch layout).
     Returns {"choice", "default", "predicted_ms", "buckets": [...]} and, with
     apply, leaves the chosen setting on the context."""
+    t0 = time.perf_counter()
     len1 = np.asarray(len1, np.int32)
     len2 = np.asarray(len2, np.int32)
     longer = np.maximum(len1, len2)
@@ -82,6 +86,7 @@ def tune_pairs(ctx: Context, op: str, blob, off1, off2, len1, len2, sample=192, 
     keys = sorted({k for c in cands for k in c})
     total = {i: 0.0 for i in range(len(cands))}
     buckets = []
+    d_blob = torch.from_numpy(np.asarray(blob)).to(device)
     try:
         for lo, hi in zip(EDGES[:-1], EDGES[1:]):
             members = np.nonzero((longer >= lo) & (longer < hi))[0]
@@ -92,7 +97,7 @@ def tune_pairs(ctx: Context, op: str, blob, off1, off2, len1, len2, sample=192, 
             ms = []
             for cand in cands:
                 set_knobs(ctx, cand)
-                ms.append(_time_fill(ctx, op, blob, off1, off2, len1, len2, idx, repeats, device))
+                ms.append(_time_fill(ctx, op, d_blob, off1, off2, len1, len2, idx, repeats, device))
             for i, t in enumerate(ms):
                 total[i] += t * scale
             best = int(np.argmin(ms))
@@ -103,9 +108,12 @@ def tune_pairs(ctx: Context, op: str, blob, off1, off2, len1, len2, sample=192, 
     finally:
         set_knobs(ctx, {k: default[k] for k in keys})
     pick = min(total, key=total.get)
+    dflt = [i for i, c in enumerate(cands) if all(default[k] == v for k, v in c.items())]
+    if dflt and total[pick] > (1.0 - margin) * total[dflt[0]]:
+        pick = dflt[0]
     choice = {k: cands[pick][k] for k in keys}
     if apply:
         set_knobs(ctx, choice)
     return {"choice": choice, "default": {k: default[k] for k in keys},
             "candidates": cands, "predicted_ms": {str(i): round(t, 3) for i, t in total.items()},
-            "buckets": buckets}
+            "buckets": buckets, "tune_s": round(time.perf_counter() - t0, 3)}
