@@ -52,6 +52,9 @@ def parse():
     ap.add_argument("--c1-pairs", type=int, default=1024)
     ap.add_argument("--c4-pairs", type=int, default=2048)
     ap.add_argument("--c5-triples", type=int, default=16384)
+    ap.add_argument("--only", choices=["c1", "c4", "c5"], default=None,
+                    help="run one secondary configuration alone (for profiling) and print "
+                         "its record")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default tuning knobs instead of autotuning config 2")
     ap.add_argument("--graph", type=int, default=0,
@@ -198,7 +201,7 @@ OPS_LINEAR = 6    # algn_fill_row: 3 candidates x (add + compare)
 OPS_CUBE = 14     # algn_fill_cube: 7 candidates x (add + compare)
 
 
-def secondary(args, pool, procs):
+def secondary(args, pool, procs, which=("c1", "c4", "c5")):
     """The other workloads at their full sizes, batched, inputs resident:
     config 1 (1,024 linear pairs, align_2 + ancestor + median with gaps),
     config 4 (2,048 pairs of 16k-100k, linear sub 1 / indel 1, band
@@ -215,6 +218,8 @@ def secondary(args, pool, procs):
     for cfg, n, tcm, steps, cpu_stride, cpu_procs in (
             (1, args.c1_pairs, (1, 2), 3, 1, procs),
             (4, args.c4_pairs, (1, 1), 1, 32, max(1, procs // 2))):
+        if f"c{cfg}" not in which:
+            continue
         print(f"[bench] config {cfg}: {n} pairs", file=sys.stderr, flush=True)
         ctx = Context(0)
         ctx.set_cm(cost_matrix.of_transformations_and_gaps(*tcm))
@@ -256,6 +261,8 @@ def secondary(args, pool, procs):
                 pool, cpu_procs, "linear", tcm, blob, [off[0::2], off[1::2]], [l1, l2],
                 sample, gpu_costs, f"every {cpu_stride}th pair" if cpu_stride > 1 else "all pairs")
         out[f"config{cfg}"] = rec
+    if "c5" not in which:
+        return out
     print(f"[bench] config 5: {args.c5_triples} triples", file=sys.stderr, flush=True)
     ctx = Context(0)
     two = cost_matrix.of_transformations_and_gaps(1, 1)
@@ -348,11 +355,31 @@ def main():
         import multiprocessing as mp
         pool = mp.get_context("fork").Pool(procs)
     try:
-        run_bench(args, pool, procs)
+        if args.only:
+            rec = secondary(args, pool, procs, which=(args.only,))
+            if args.copy_calib_gb > 0:
+                rec["copy_calibration_bytes"] = copy_calibration(args.copy_calib_gb)
+            print(json.dumps(rec), flush=True)
+        else:
+            run_bench(args, pool, procs)
     finally:
         if pool is not None:
             pool.close()
             pool.join()
+
+
+def copy_calibration(gb):
+    """A device-to-device copy of gb GiB: the scale of the FETCH_SIZE and
+    WRITE_SIZE counters against known bytes."""
+    import torch
+    nb = int(gb * (1 << 30))
+    x = torch.ones(nb, dtype=torch.uint8, device="cuda:0")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    y.copy_(x)
+    torch.cuda.synchronize()
+    del x, y
+    return nb
 
 
 def run_bench(args, pool, procs):
@@ -492,14 +519,7 @@ def run_bench(args, pool, procs):
         print("[bench] serving: 128-taxon SPR rejoin traffic", file=sys.stderr, flush=True)
         result["serving"] = serving(args.serve_requests)
     if args.copy_calib_gb > 0:
-        nb = int(args.copy_calib_gb * (1 << 30))
-        x = torch.ones(nb, dtype=torch.uint8, device=dev)
-        y = torch.empty_like(x)
-        torch.cuda.synchronize()
-        y.copy_(x)
-        torch.cuda.synchronize()
-        result["copy_calibration_bytes"] = nb
-        del x, y
+        result["copy_calibration_bytes"] = copy_calibration(args.copy_calib_gb)
     ctx.close()
     print(json.dumps(result), flush=True)
 
