@@ -84,6 +84,31 @@ def test_affine_unit_hand_entries():
     assert med[((A | C) << 5) + (G | T)] == A | C | G | T
 
 
+def test_linear_hand_set_medians_tcm12():
+    """Hand-derived set entries of tcm (1, 2), linear model: for sets I, J
+    the cost is the least cost(a, v) + cost(v, b) over a in I, b in J and a
+    state v, and the median is the union of every v reaching it
+    (test_combinations, src/cost_matrix.ml:479-505, filled at 862-897; no
+    cleanup for the linear model, 700-704).  With substitution 1 and gap 2
+    a base pair (a, b) reaches 1 through v = a or v = b, a shared state
+    reaches 0 through itself, and (x, gap) reaches 2 through x or gap."""
+    entries = [
+        (A, C | G, 1, A | C | G),
+        (A | C, A | G, 0, A),
+        (A | GAP, C, 1, A | C),
+        (A | GAP, C | GAP, 0, GAP),
+        (A | C, G | T, 1, A | C | G | T),
+        (A | C | G, T, 1, A | C | G | T),
+        (C, G | T | GAP, 1, C | G | T),
+        (A | C, GAP, 2, A | C | GAP),
+    ]
+    for cm in (oracle.cm_dna(1, 2), cost_matrix.of_transformations_and_gaps(1, 2)):
+        cost, med = np.asarray(cm.cost), np.asarray(cm.median)
+        for i, j, c, m in entries:
+            assert cost[(i << 5) + j] == c, (i, j)
+            assert med[(i << 5) + j] == m, (i, j)
+
+
 @pytest.mark.parametrize("args", [(1, 2, None), (1, 1, None), (1, 1, 3), (2, 3, None), (2, 1, 2)])
 def test_product_tables_match_oracle(args):
     """poy5_amd.cost_matrix (Python) and the oracle (C) are separate
