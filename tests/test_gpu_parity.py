@@ -328,6 +328,31 @@ def test_batch_linear_config4_long(ctx):
         assert np.array_equal(outs[k][3], oracle.median_2_with_gaps(o1, o2, ocm)), k
 
 
+def test_batch_linear_config4_beyond_int_indexing(ctx):
+    """The three largest of 24 config-4 draws (L1 * L2 above 2^31, where
+    the reference's int products at src/algn.c:985 and :1019 overflow):
+    size-independent checks.  align_2 and cost_2 agree, the aligned pair
+    re-prices to the cost, drops back to the inputs, and its ancestor and
+    median with gaps equal the oracle's medians of that aligned pair."""
+    say("test_batch_linear_config4_beyond_int_indexing")
+    ctx.set_cm(cost_matrix.of_transformations_and_gaps(1, 1))
+    ocm = oracle.cm_dna(1, 1)
+    pairs = workloads.pairs_list(4, 24)
+    pairs.sort(key=lambda p: len(p[0]) * len(p[1]))
+    pairs = pairs[-3:]
+    assert all(len(a) * len(b) > 2 ** 31 for a, b in pairs)
+    costs, outs = run_pairs(ctx, "align_2d", pairs)
+    c2, _ = run_pairs(ctx, "cost_2d", pairs)
+    assert np.array_equal(costs, c2)
+    for k, (a, b) in enumerate(pairs):
+        r1, r2, anc, mwg = outs[k]
+        assert len(r1) == len(r2)
+        assert oracle.verify_2(r1, r2, ocm) == costs[k], k
+        assert np.array_equal(r1[r1 != 16], a[a != 16]) and np.array_equal(r2[r2 != 16], b[b != 16])
+        assert np.array_equal(anc, oracle.ancestor_2(r1, r2, ocm)), k
+        assert np.array_equal(mwg, oracle.median_2_with_gaps(r1, r2, ocm)), k
+
+
 def test_batch_linear_two_phase(ctx):
     """A direction-store budget far below the full-width estimate: cost-only
     rounds for every pair, then final rounds in chunks sized to the bands.

@@ -424,3 +424,18 @@ def test_3d_one_row_planes_all_read_plane_zero():
         for n in (2, 3, 6):
             s1 = np.concatenate([rseq(rng, n - 1, 0.3), [last]]).astype(np.uint8)
             assert oracle.nw_3d(s1, s2, s3, c3)[0] == ref
+
+
+def test_median_3_repeats_the_last_column():
+    """algn_get_median_3d (src/algn.c:3641-3654) never moves its end
+    pointers: the median has len(s1) entries, each the 3D median of the
+    last column."""
+    c3 = oracle.cm3_from_2d(oracle.cm_dna(1, 2))
+    med = np.array(c3.median)
+    rng = np.random.default_rng(18)
+    for _ in range(60):
+        n = int(rng.integers(1, 12))
+        a, b, c = (rng.integers(1, 32, n).astype(np.uint8) for _ in range(3))
+        want = med[(int(a[-1]) << 10) + (int(b[-1]) << 5) + int(c[-1])]
+        got = oracle.median_3(a, b, c, c3)
+        assert len(got) == n and np.all(got == want)
