@@ -86,6 +86,29 @@ int poyhip_backtrace_2d(poyhip_ctx *ctx, const uint8_t *s1, int l1,
                         const uint8_t *s2, int l2, uint8_t *r1, uint8_t *r2,
                         int cap, int *len, int swaped);
 
+/* algn_CAML_limit_2 (src/algn.c:3161) = Align.c_cost_2_limit
+ * (src/sequence.ml:620), called by Align.cost_2_limit (764).  The reference
+ * fills the whole sequences (s1 not longer than s2, as poyhip_cost_2): the
+ * limits enter only the full-plane-vs-band decision, whose height becomes
+ * 50 + (s1_len - s2_len) + w (algn_nw_limit, src/algn.c:2954); h, s1_st and
+ * s2_st are read nowhere.  Leaves the direction store for
+ * poyhip_backtrace_2d_limit / poyhip_backtrace_2d like poyhip_cost_2. */
+int poyhip_limit_2(poyhip_ctx *ctx, const uint8_t *s1, int l1, const uint8_t *s2, int l2,
+                   int w, int h, int s1_st, int s2_st, int s1_len, int s2_len, int *cost);
+
+/* algn_CAML_backtrace_2d_limit (src/algn.c:3449) = Align.extract_edited_2_limit
+ * (src/sequence.ml:948): backtrace_2d (3278) of the last fill over a
+ * window.  The walk starts at row algn_s1 - 1, column st_s2 + algn_s2 - 1,
+ * reads s1 from index st_s1 + algn_s1 - 1 down (st_s1 shifts the symbols,
+ * not the row, as in the reference) and s2 from st_s2 + algn_s2 - 1 down,
+ * and ends when it reaches row 0 left of column st_s2.  Limits outside the
+ * sequences: POYHIP_EINVAL.  Defined while the walk stays inside the band
+ * the last fill kept (always after a full-plane fill); outside it the
+ * reference reads a direction matrix that fill never wrote. */
+int poyhip_backtrace_2d_limit(poyhip_ctx *ctx, const uint8_t *s1, int l1, const uint8_t *s2,
+                              int l2, uint8_t *r1, uint8_t *r2, int cap, int *len, int st_s1,
+                              int st_s2, int algn_s1, int algn_s2, int swaped);
+
 /* algn_CAML_align_2d (src/algn.c:3503) = Align.c_align_2: fill + backtrace. */
 int poyhip_align_2d(poyhip_ctx *ctx, const uint8_t *s1, int l1, const uint8_t *s2,
                     int l2, int deltawh, int swaped, uint8_t *r1, uint8_t *r2,

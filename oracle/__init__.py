@@ -68,6 +68,10 @@ def lib():
         L.orc_nw.argtypes = [u8p, C.c_int, u8p, C.c_int, C.POINTER(OrcCM), C.c_int,
                              u16p, C.POINTER(OrcStats)]
         L.orc_backtrace_2d.argtypes = [u8p, C.c_int, u8p, C.c_int, u16p, C.c_int, u8p, u8p]
+        L.orc_nw_limit.argtypes = [u8p, C.c_int, u8p, C.c_int, C.POINTER(OrcCM), C.c_int,
+                                   C.c_int, C.c_int, u16p, C.POINTER(OrcStats)]
+        L.orc_backtrace_2d_limit.argtypes = [u8p, C.c_int, u8p, C.c_int, u16p, C.c_int, C.c_int,
+                                             C.c_int, C.c_int, C.c_int, u8p, u8p]
         L.orc_backtrace_2d_gaps.argtypes = [u16p, C.c_int, C.c_int]
         for f in ("orc_ancestor_2", "orc_median_2_no_gaps", "orc_median_2_with_gaps"):
             getattr(L, f).argtypes = [u8p, u8p, C.c_int, C.POINTER(OrcCM), u8p]
@@ -135,6 +139,33 @@ def nw(s1, s2, cm, deltawh):
     c = lib().orc_nw(ap, len(a), bp, len(b), C.byref(cm), deltawh,
                      dm.ctypes.data_as(C.POINTER(C.c_uint16)), C.byref(st))
     return c, dm, st
+
+
+def limit_2(s1, s2, cm, w, s1_len, s2_len):
+    """algn_CAML_limit_2 (h, s1_st and s2_st are read nowhere): returns
+    (cost, dm[L1,L2] uint16, stats)."""
+    a, ap = _u8(s1)
+    b, bp = _u8(s2)
+    dm = np.zeros((len(a), len(b)), dtype=np.uint16)
+    st = OrcStats()
+    c = lib().orc_nw_limit(ap, len(a), bp, len(b), C.byref(cm), w, s1_len, s2_len,
+                           dm.ctypes.data_as(C.POINTER(C.c_uint16)), C.byref(st))
+    return c, dm, st
+
+
+def backtrace_2d_limit(s1, s2, dm, st_s1, st_s2, algn_s1, algn_s2, swaped):
+    """algn_CAML_backtrace_2d_limit: the aligned pair of the limited walk."""
+    a, ap = _u8(s1)
+    b, bp = _u8(s2)
+    cap = len(a) + len(b)
+    r1 = np.zeros(cap, np.uint8)
+    r2 = np.zeros(cap, np.uint8)
+    n = lib().orc_backtrace_2d_limit(ap, len(a), bp, len(b),
+                                     np.ascontiguousarray(dm).ctypes.data_as(C.POINTER(C.c_uint16)),
+                                     st_s1, st_s2, algn_s1, algn_s2, int(swaped),
+                                     r1.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                     r2.ctypes.data_as(C.POINTER(C.c_uint8)))
+    return r1[:n].copy(), r2[:n].copy()
 
 
 def backtrace_2d(s1, s2, dm, swaped):

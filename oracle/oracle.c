@@ -352,16 +352,25 @@ static int newkk_test(const uint8_t *s1, int L1, int L2, const int *prec,
 
 int orc_nw(const uint8_t *s1, int L1, const uint8_t *s2, int L2,
            const orc_cm *cm, int deltawh, uint16_t *dm, orc_stats *st) {
+    /* algn_nw (2972): the limits are the whole sequences */
+    return orc_nw_limit(s1, L1, s2, L2, cm, deltawh, L1, L2, dm, st);
+}
+
+int orc_nw_limit(const uint8_t *s1, int L1, const uint8_t *s2, int L2,
+                 const orc_cm *cm, int deltawh, int len_s1, int len_s2, uint16_t *dm,
+                 orc_stats *st) {
     orc_stats local;
     if (!st) st = &local;
     memset(st, 0, sizeof *st);
     st->min_gap_excess = INT_MAX;
     int *prec = precalc_2d(cm, s2, L2);
     int *mm = (int *)malloc(sizeof(int) * (size_t)L1 * L2);
-    /* algn_nw_limit (2954): width 50, height 50 + (L1 - L2) */
+    /* algn_nw_limit (2954) -> algn_fill_plane_2 (1134): width 50, height
+     * 50 + (len_s1 - len_s2), each widened by deltawh; the whole sequences
+     * are filled whatever the limits */
     int width = 50 + deltawh;
     if (width > L2) width = L2;
-    int height = 50 + (L1 - L2) + deltawh;
+    int height = 50 + (len_s1 - len_s2) + deltawh;
     if (height > L1) height = L1;
     (void)width;
     int delta = orc_min_non0_cost(cm);
@@ -421,13 +430,25 @@ static int rs_out(rseq *r, uint8_t *out) {
 
 int orc_backtrace_2d(const uint8_t *s1, int L1, const uint8_t *s2, int L2,
                      const uint16_t *dm, int swaped, uint8_t *r1, uint8_t *r2) {
-    /* backtrace_2d (3278-3327) + follow_* (79-124) */
+    /* algn_CAML_backtrace_2d (3424): no limits */
+    return orc_backtrace_2d_limit(s1, L1, s2, L2, dm, 0, 0, L1, L2, swaped, r1, r2);
+}
+
+int orc_backtrace_2d_limit(const uint8_t *s1, int L1, const uint8_t *s2, int L2,
+                           const uint16_t *dm, int st_s1, int st_s2, int algn_s1, int algn_s2,
+                           int swaped, uint8_t *r1, uint8_t *r2) {
+    /* backtrace_2d (3278-3327) + follow_* (79-124).  The walk's pointer
+     * starts at beg + L2 (algn_s1 - 1) + algn_s2 - 1 with beg = dm + st_s2
+     * and runs while it stays at or above beg; the symbol indices start at
+     * algn_s1 + st_s1 and algn_s2 + st_s2 (st_s1 moves the symbols read,
+     * not the row). */
+    (void)L1;
     rseq a, b;
     rs_init(&a, L1 + L2);
     rs_init(&b, L1 + L2);
-    long end = (long)L2 * (L1 - 1) + L2 - 1;
-    int i = L1, j = L2;
-    while (end >= 0) {
+    long end = (long)st_s2 + (long)L2 * (algn_s1 - 1) + algn_s2 - 1;
+    int i = algn_s1 + st_s1, j = algn_s2 + st_s2;
+    while (end >= st_s2) {
         uint16_t f = dm[end];
         if (f & ORC_ALIGN) {
             i--; rs_prepend(&a, s1[i]);

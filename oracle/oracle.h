@@ -106,10 +106,19 @@ int orc_min_non0_cost(const orc_cm *cm);
  * like the reference's direction matrix.  Returns the cost. */
 int orc_nw(const uint8_t *s1, int L1, const uint8_t *s2, int L2,
            const orc_cm *cm, int deltawh, uint16_t *dm, orc_stats *st);
+/* algn_nw_limit (2954) as algn_CAML_limit_2 (3161) calls it: the limit
+ * lengths enter only the full-plane-vs-band decision's height. */
+int orc_nw_limit(const uint8_t *s1, int L1, const uint8_t *s2, int L2,
+                 const orc_cm *cm, int deltawh, int len_s1, int len_s2, uint16_t *dm,
+                 orc_stats *st);
 /* backtrace_2d (3278).  r1/r2 receive the aligned sequences left to right;
  * returns the aligned length. */
 int orc_backtrace_2d(const uint8_t *s1, int L1, const uint8_t *s2, int L2,
                      const uint16_t *dm, int swaped, uint8_t *r1, uint8_t *r2);
+/* backtrace_2d with the limits of algn_CAML_backtrace_2d_limit (3449). */
+int orc_backtrace_2d_limit(const uint8_t *s1, int L1, const uint8_t *s2, int L2,
+                           const uint16_t *dm, int st_s1, int st_s2, int algn_s1, int algn_s2,
+                           int swaped, uint8_t *r1, uint8_t *r2);
 /* backtrace_2d_gaps (978) */
 int orc_backtrace_2d_gaps(const uint16_t *dm, int L1, int L2);
 /* algn_ancestor_2 (3603); returns median length */

@@ -73,6 +73,11 @@ SIGNATURES = {
     "poyhip_cost_2": (C.c_int, [_vp, _u8p, C.c_int, _u8p, C.c_int, C.c_int, _ip]),
     "poyhip_backtrace_2d": (C.c_int, [_vp, _u8p, C.c_int, _u8p, C.c_int, _u8p, _u8p,
                                       C.c_int, _ip, C.c_int]),
+    "poyhip_limit_2": (C.c_int, [_vp, _u8p, C.c_int, _u8p, C.c_int, C.c_int, C.c_int, C.c_int,
+                                 C.c_int, C.c_int, C.c_int, _ip]),
+    "poyhip_backtrace_2d_limit": (C.c_int, [_vp, _u8p, C.c_int, _u8p, C.c_int, _u8p, _u8p,
+                                            C.c_int, _ip, C.c_int, C.c_int, C.c_int, C.c_int,
+                                            C.c_int]),
     "poyhip_align_2d": (C.c_int, [_vp, _u8p, C.c_int, _u8p, C.c_int, C.c_int, C.c_int,
                                   _u8p, _u8p, C.c_int, _ip, _ip]),
     "poyhip_cost_affine_3": (C.c_int, [_vp, _u8p, C.c_int, _u8p, C.c_int, _ip]),

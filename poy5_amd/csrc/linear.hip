@@ -278,6 +278,11 @@ struct LinBtArgs {
   const PairResult* res;
   uint8_t* out;
   int* outlen;  // 4 per pair: r1, r2, ancestor, median with gaps
+  // limits of algn_CAML_backtrace_2d_limit (one pair; lim = 0: none): the
+  // walk starts at row a1 - 1, column st2 + a2 - 1, reads the rows' symbols
+  // from st1 + a1 - 1 down and the columns' from st2 + a2 - 1 down, and
+  // ends when it reaches row 0 left of column st2
+  int lim = 0, lim_st1 = 0, lim_st2 = 0, lim_a1 = 0, lim_a2 = 0;
 };
 
 // backtrace_2d (src/algn.c:3278) over the packed moves, emitting the aligned
@@ -329,10 +334,13 @@ __global__ __launch_bounds__(64) void k_linear_backtrace(LinBtArgs A) {
     n++;
     if ((n & 63) == 0) r1[cap - 1 - (n - 64) - lane] = (uint8_t)mv;
   };
-  int i = L1 - 1, j = L2 - 1;
+  int i = A.lim ? A.lim_a1 - 1 : L1 - 1, j = A.lim ? A.lim_st2 + A.lim_a2 - 1 : L2 - 1;
+  const int stop_col = A.lim ? A.lim_st2 : 0;
+  const int top_a = A.lim ? A.lim_st1 + A.lim_a1 - 1 : L1 - 1;  // last row symbol read
+  const int top_b = A.lim ? A.lim_st2 + A.lim_a2 - 1 : L2 - 1;
   int cb = -1, jlo = 0;
   long long base = 0;  // first 16-step block of the stripe
-  while (i != 0 && j != 0) {
+  while (i > 0 && j > 0) {
     const int bb = (i - 1) >> 6;
     if (bb != cb) {
       cb = bb;
@@ -379,11 +387,14 @@ __global__ __launch_bounds__(64) void k_linear_backtrace(LinBtArgs A) {
       j--;
     }
   }
-  // the reference's border moves: up along column 0, left along row 0, and
-  // the (0, 0) cell
-  while (i != 0) { record(1); i--; }
-  while (j != 0) { record(2); j--; }
-  record(0);
+  // the reference's border moves: up along column 0, left along row 0 (to
+  // column stop_col), and the (0, 0) cell; a walk that starts above row 0
+  // (a limit of a1 = 0) makes no move
+  if (i >= 0) {
+    while (i > 0) { record(1); i--; }
+    while (j > 0 && j >= stop_col) { record(2); j--; }
+    if (j == 0 && stop_col == 0) record(0);
+  }
   if ((n & 63) != 0 && lane < (n & 63)) r1[cap - 1 - (n & ~63) - lane] = (uint8_t)mv;
   __threadfence_block();
 
@@ -397,8 +408,8 @@ __global__ __launch_bounds__(64) void k_linear_backtrace(LinBtArgs A) {
     // moves before this one, packed: rows in bits 0-7, columns in 8-15
     int tot;
     const int ex = wave_excl_sum((di ? 1 : 0) | (dj ? 256 : 0), tot);
-    const int x = di ? sa[L1 - 1 - ci - (ex & 255)] : kGap;
-    const int y = dj ? sb[L2 - 1 - cj - (ex >> 8)] : kGap;
+    const int x = di ? sa[top_a - ci - (ex & 255)] : kGap;
+    const int y = dj ? sb[top_b - cj - (ex >> 8)] : kGap;
     const int md = sw ? s_med[(y << 5) + x] : s_med[(x << 5) + y];
     const bool ea = ok && md != kGap;
     int atot;

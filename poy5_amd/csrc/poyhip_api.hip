@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <climits>
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
@@ -280,7 +281,7 @@ hipError_t launch_linear_fill(poyhip_ctx* c, const LinFillArgs& A, bool dm, int 
 
 int run_linear_one(poyhip_ctx* c, const uint8_t* s1, int l1, const uint8_t* s2, int l2,
                    int deltawh, int prio, bool backtrace, PairResult* out_res,
-                   std::vector<uint8_t>* outblk, int outlen[4]) {
+                   std::vector<uint8_t>* outblk, int outlen[4], const int* limit = nullptr) {
   PairTable t;
   int* counter;
   int rc = stage_pair(c, s1, l1, s2, l2, prio, deltawh, &t, &counter);
@@ -315,6 +316,13 @@ int run_linear_one(poyhip_ctx* c, const uint8_t* s1, int l1, const uint8_t* s2, 
     B.res = dres;
     B.out = dout;
     B.outlen = doutlen;
+    if (limit) {
+      B.lim = 1;
+      B.lim_st1 = limit[0];
+      B.lim_st2 = limit[1];
+      B.lim_a1 = limit[2];
+      B.lim_a2 = limit[3];
+    }
     hipLaunchKernelGGL(k_linear_backtrace, dim3(1), dim3(64), 0, c->stream, B);
     HIPCHK(c, hipGetLastError());
     back = kResBytes + outb;
@@ -445,6 +453,44 @@ int poyhip_cost_2(poyhip_ctx* c, const uint8_t* s1, int l1, const uint8_t* s2, i
   c->last_deltawh = deltawh;
   c->last_s1.assign(s1, s1 + l1);
   c->last_s2.assign(s2, s2 + l2);
+  return POYHIP_OK;
+}
+
+int poyhip_limit_2(poyhip_ctx* c, const uint8_t* s1, int l1, const uint8_t* s2, int l2, int w,
+                   int h, int s1_st, int s2_st, int s1_len, int s2_len, int* cost) {
+  (void)h;      // algn_CAML_limit_2 reads neither h
+  (void)s1_st;  // nor the start positions
+  (void)s2_st;
+  // algn_nw_limit's height is 50 + (s1_len - s2_len) + w where algn_nw's is
+  // 50 + (l1 - l2) + deltawh: the same decision with this deltawh
+  const int64_t dw = (int64_t)w + ((int64_t)s1_len - s2_len) - ((int64_t)l1 - l2);
+  if (dw < INT_MIN || dw > INT_MAX) return c ? set_err(c, POYHIP_EINVAL, "limits out of range") : POYHIP_EINVAL;
+  return poyhip_cost_2(c, s1, l1, s2, l2, (int)dw, cost);
+}
+
+int poyhip_backtrace_2d_limit(poyhip_ctx* c, const uint8_t* s1, int l1, const uint8_t* s2,
+                              int l2, uint8_t* r1, uint8_t* r2, int cap, int* len, int st_s1,
+                              int st_s2, int algn_s1, int algn_s2, int swaped) {
+  if (!c || !r1 || !r2 || !len || !valid_seq(s1, l1) || !valid_seq(s2, l2)) return POYHIP_EINVAL;
+  if (st_s1 < 0 || st_s2 < 0 || algn_s1 < 0 || algn_s2 < 0 || st_s1 + algn_s1 > l1 ||
+      st_s2 + algn_s2 > l2)
+    return set_err(c, POYHIP_EINVAL, "limits outside the sequences");
+  if (!c->last2d || (int)c->last_s1.size() != l1 || (int)c->last_s2.size() != l2 ||
+      memcmp(c->last_s1.data(), s1, l1) || memcmp(c->last_s2.data(), s2, l2))
+    return set_err(c, POYHIP_ESTATE, "backtrace without a matching poyhip_cost_2 / poyhip_limit_2");
+  if (cap < l1 + l2) return set_err(c, POYHIP_ECAP, "capacity %d < %d", cap, l1 + l2);
+  (void)hipSetDevice(c->device);
+  PairResult r;
+  int outlen[4];
+  std::vector<uint8_t> blk;
+  const int limit[4] = {st_s1, st_s2, algn_s1, algn_s2};
+  int rc = run_linear_one(c, s1, l1, s2, l2, c->last_deltawh, swaped ? 1 : 0, true, &r, &blk,
+                          outlen, limit);
+  if (rc) return rc;
+  const int cp = (int)lin_out_cap(l1, l2);
+  *len = outlen[0];
+  memcpy(r1, blk.data() + cp - outlen[0], outlen[0]);
+  memcpy(r2, blk.data() + cp + cp - outlen[1], outlen[1]);
   return POYHIP_OK;
 }
 

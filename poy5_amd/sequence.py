@@ -101,6 +101,52 @@ class Align:
             r2.ctypes.data_as(C.POINTER(C.c_uint8)), cap, C.byref(n), int(swaped)))
         return r1[:n.value].copy(), r2[:n.value].copy()
 
+    def c_cost_2_limit(self, s1, s2, w: int, h: int, s1_st: int, s2_st: int, s1_len: int,
+                       s2_len: int) -> int:
+        """algn_CAML_limit_2: s1 must not be longer than s2; the limits only
+        move the full-plane-vs-band decision (h, s1_st, s2_st are unused)."""
+        a, ap = _u8(s1)
+        b, bp = _u8(s2)
+        out = C.c_int()
+        self.ctx.chk(self.L.poyhip_limit_2(self.ctx.h, ap, len(a), bp, len(b), w, h, s1_st,
+                                           s2_st, s1_len, s2_len, C.byref(out)))
+        return out.value
+
+    def extract_edited_2_limit(self, s1, s2, st_s1: int, st_s2: int, algn_s1: int,
+                               algn_s2: int, swaped: int):
+        """algn_CAML_backtrace_2d_limit after c_cost_2 / c_cost_2_limit on
+        the same sequences: the walk over the window (see poyhip.h)."""
+        a, ap = _u8(s1)
+        b, bp = _u8(s2)
+        cap = len(a) + len(b)
+        r1 = np.zeros(cap, np.uint8)
+        r2 = np.zeros(cap, np.uint8)
+        n = C.c_int()
+        self.ctx.chk(self.L.poyhip_backtrace_2d_limit(
+            self.ctx.h, ap, len(a), bp, len(b), r1.ctypes.data_as(C.POINTER(C.c_uint8)),
+            r2.ctypes.data_as(C.POINTER(C.c_uint8)), cap, C.byref(n), st_s1, st_s2, algn_s1,
+            algn_s2, int(swaped)))
+        return r1[:n.value].copy(), r2[:n.value].copy()
+
+    def cost_2_limit(self, s1, s2, a: int, b: int, c: int, d: int, w: int | None = None,
+                     h: int | None = None) -> int:
+        """Align.cost_2_limit (sequence.ml:748-783): default width and height
+        min(d, 2 |d - c|) clamped to c and d, positions a of s1 and b of s2
+        set to the gap during the call, and the sequence with the longer
+        limit passed first, as the OCaml does."""
+        default = min(d, 2 * abs(d - c))
+        w = default if w is None else w
+        h = default if h is None else h
+        w = c if w > c else w
+        h = d if h > d else h
+        s1 = np.array(s1, np.uint8)
+        s2 = np.array(s2, np.uint8)
+        s1[a] = GAP
+        s2[b] = GAP
+        if c >= d:
+            return self.c_cost_2_limit(s1, s2, w, h, a, b, c, d)
+        return self.c_cost_2_limit(s2, s1, h, w, b, a, d, c)
+
     def cost_2_affine(self, s1, s2) -> int:
         """algn_CAML_cost_affine_3: full affine DP, either order."""
         a, ap = _u8(s1)

@@ -369,3 +369,53 @@ def test_batch_linear_two_phase(ctx):
             assert np.array_equal(o1[k][r], o2[k][r]), (k, r)
         r1, r2, oc, _ = oracle.align_2(a, b, ocm)
         assert c2[k] == oc and np.array_equal(o2[k][0], r1) and np.array_equal(o2[k][1], r2), k
+
+
+def test_limit_2_and_backtrace_2d_limit_dropins(ctx):
+    """algn_CAML_limit_2 and algn_CAML_backtrace_2d_limit through the C ABI
+    against the oracle: costs for limits on both sides of the
+    full-plane-vs-band decision, and windowed walks (start cell, st_s2 stop,
+    st_s1 symbol shift) on full-plane fills, both swap orders; the unlimited
+    window equals backtrace_2d on a banded fill too."""
+    say("test_limit_2_and_backtrace_2d_limit_dropins")
+    from poy5_amd.sequence import Align
+    ctx.set_cm(cost_matrix.of_transformations_and_gaps(1, 2))
+    ocm = oracle.cm_dna(1, 2)
+    al = Align(ctx)
+    rng = np.random.default_rng(0x11A)
+    for t in range(40):
+        n1 = int(rng.integers(2, 40))
+        n2 = int(rng.integers(n1, 48))
+        a = np.concatenate([[16], rng.choice(PLAIN, n1 - 1)]).astype(np.uint8)
+        b = np.concatenate([[16], rng.choice(PLAIN, n2 - 1)]).astype(np.uint8)
+        s1_len, s2_len = int(rng.integers(0, 400)), int(rng.integers(0, 400))
+        w = int(rng.integers(0, 30))
+        c = al.c_cost_2_limit(a, b, w, 7, 1, 2, s1_len, s2_len)
+        oc, odm, ost = oracle.limit_2(a, b, ocm, w, s1_len, s2_len)
+        assert c == oc, t
+        if not ost.full_plane:
+            continue
+        for _ in range(6):
+            st1 = int(rng.integers(0, n1))
+            a1 = int(rng.integers(0, n1 - st1 + 1))
+            st2 = int(rng.integers(0, n2))
+            a2 = int(rng.integers(0, n2 - st2 + 1))
+            sw = int(rng.integers(0, 2))
+            r1, r2 = al.extract_edited_2_limit(a, b, st1, st2, a1, a2, sw)
+            o1, o2 = oracle.backtrace_2d_limit(a, b, odm, st1, st2, a1, a2, sw)
+            assert np.array_equal(r1, o1) and np.array_equal(r2, o2), (t, st1, st2, a1, a2, sw)
+    # banded fill, whole window = backtrace_2d
+    root = np.concatenate([[16], rng.choice(PLAIN, 600)]).astype(np.uint8)
+    a, b = mutate(rng, root, 0.03), mutate(rng, root, 0.03)
+    if len(a) > len(b):
+        a, b = b, a
+    c = al.c_cost_2_limit(a, b, 0, 0, 0, 0, len(a), len(b))
+    oc, odm, ost = oracle.limit_2(a, b, ocm, 0, len(a), len(b))
+    assert c == oc and not ost.full_plane
+    for sw in (0, 1):
+        r1, r2 = al.extract_edited_2_limit(a, b, 0, 0, len(a), len(b), sw)
+        o1, o2 = oracle.backtrace_2d(a, b, odm, sw)
+        assert np.array_equal(r1, o1) and np.array_equal(r2, o2), sw
+    from poy5_amd._lib import PoyhipError
+    with pytest.raises(PoyhipError):
+        al.extract_edited_2_limit(a, b, 1, 0, len(a), len(b), 0)

@@ -439,3 +439,94 @@ def test_median_3_repeats_the_last_column():
         want = med[(int(a[-1]) << 10) + (int(b[-1]) << 5) + int(c[-1])]
         got = oracle.median_3(a, b, c, c3)
         assert len(got) == n and np.all(got == want)
+
+
+def _moves_of(r1, r2):
+    """Walk moves of an aligned pair without gap-bit symbols, last column
+    first: 0 align (both advance), 1 row only, 2 column only."""
+    out = []
+    for x, y in zip(reversed(ints(r1)), reversed(ints(r2))):
+        if x == 16 and y == 16:
+            out.append(0)  # the leading gaps, cell (0, 0)
+        elif y == 16:
+            out.append(1)
+        elif x == 16:
+            out.append(2)
+        else:
+            out.append(0)
+    return out
+
+
+@pytest.mark.parametrize("tcm", [(1, 2), (2, 1), (1, 1)])
+def test_limit_walk_from_any_cell_is_the_prefix_alignment(tcm):
+    """backtrace_2d with the limits of algn_CAML_backtrace_2d_limit: from
+    cell (r, c) of a full-plane matrix (st_s1 = st_s2 = 0, algn = r+1, c+1)
+    the walk is the tie-exact alignment of the prefixes s1[:r+1], s2[:c+1]
+    (the fill is prefix-closed), for both swap orders."""
+    cm = oracle.cm_dna(*tcm)
+    C = np.array(cm.cost)
+    rng = np.random.default_rng(900 + 10 * tcm[0] + tcm[1])
+    for _ in range(50):
+        n1 = int(rng.integers(1, 6))
+        n2 = int(rng.integers(n1, 7))
+        a, b = rseq(rng, n1, 0.15), rseq(rng, n2, 0.15)
+        _, dm, st = oracle.nw(a, b, cm, 100)
+        assert st.full_plane
+        for r in range(len(a)):
+            for c in range(len(b)):
+                for sw in (0, 1):
+                    _, (p1, p2) = brute.best_linear(ints(a[:r + 1]), ints(b[:c + 1]), C, sw)
+                    x1, x2 = oracle.backtrace_2d_limit(a, b, dm, 0, 0, r + 1, c + 1, sw)
+                    assert ints(x1) == ints(p1) and ints(x2) == ints(p2), (r, c, sw)
+
+
+def test_limit_walk_stops_left_of_st_s2_and_st_s1_moves_symbols_only():
+    """With st_s2 = x the walk from column x + algn_s2 - 1 is the unlimited
+    walk's columns up to its first cell of row 0 left of column x; st_s1
+    changes the row symbols read, not the moves."""
+    cm = oracle.cm_dna(1, 2)
+    rng = np.random.default_rng(910)
+    for _ in range(60):
+        n1 = int(rng.integers(2, 12))
+        n2 = int(rng.integers(n1, 14))
+        a, b = rseq(rng, n1, 0.1), rseq(rng, n2, 0.1)
+        _, dm, _ = oracle.nw(a, b, cm, 100)
+        L2 = len(b)
+        r = int(rng.integers(1, len(a)))
+        c = int(rng.integers(1, len(b)))
+        sw = int(rng.integers(0, 2))
+        f1, f2 = oracle.backtrace_2d_limit(a, b, dm, 0, 0, r + 1, c + 1, sw)
+        for x in range(0, c + 1):
+            # columns of the unlimited walk before it passes below column x of row 0
+            i, j, keep = r, c, 0
+            for m in _moves_of(f1, f2):
+                if i * L2 + j < x:
+                    break
+                keep += 1
+                i, j = (i - 1, j - 1) if m == 0 else ((i - 1, j) if m == 1 else (i, j - 1))
+            g1, g2 = oracle.backtrace_2d_limit(a, b, dm, 0, x, r + 1, c + 1 - x, sw)
+            assert ints(g1) == ints(f1[len(f1) - keep:]) and ints(g2) == ints(f2[len(f2) - keep:])
+        # st_s1: the same moves, row symbols from s1[y:]
+        y = int(rng.integers(0, len(a) - r))
+        h1, h2 = oracle.backtrace_2d_limit(a, b, dm, y, 0, r + 1, c + 1, sw)
+        assert _moves_of(h1, h2) == _moves_of(f1, f2) or y > 0
+        assert ints(h2) == ints(f2)
+        rows = [v for v, m in zip(reversed(ints(h1)), _moves_of(f1, f2)) if m != 2]
+        assert rows == ints(a[y:y + r + 1])[::-1][:len(rows)]
+
+
+def test_limit_2_moves_only_the_decision():
+    """algn_nw_limit's height is 50 + (s1_len - s2_len) + w: limits that
+    raise it turn a banded fill into a full plane; the cost, optimal either
+    way, does not change."""
+    cm = oracle.cm_dna(1, 2)
+    rng = np.random.default_rng(920)
+    for _ in range(10):
+        b = rseq(rng, 320)
+        a = mutate(rng, b, 40)[:301]
+        c0, _, st0 = oracle.nw(a, b, cm, 0)
+        assert not st0.full_plane  # height 50 + (301 - 321) = 30 < 301 / 2
+        c1, _, st1 = oracle.limit_2(a, b, cm, 0, 900, 100)  # height 850, clamped to 301
+        assert st1.full_plane and c1 == c0
+        c2, _, st2 = oracle.limit_2(a, b, cm, 0, len(a), len(b))
+        assert not st2.full_plane and c2 == c0
