@@ -46,15 +46,15 @@ def parse():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-serve", action="store_true",
                     help="skip the serving run (128-taxon SPR rejoin traffic)")
-    ap.add_argument("--serve-requests", type=int, default=4000)
+    ap.add_argument("--serve-requests", type=int, default=20000)
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the secondary configurations (1, 4, 5)")
     ap.add_argument("--c1-pairs", type=int, default=1024)
     ap.add_argument("--c4-pairs", type=int, default=2048)
     ap.add_argument("--c5-triples", type=int, default=16384)
-    ap.add_argument("--only", choices=["c1", "c4", "c5"], default=None,
-                    help="run one secondary configuration alone (for profiling) and print "
-                         "its record")
+    ap.add_argument("--only", choices=["c1", "c4", "c5", "serve"], default=None,
+                    help="run one secondary configuration or the serving run alone (for "
+                         "profiling) and print its record")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default tuning knobs instead of autotuning config 2")
     ap.add_argument("--graph", type=int, default=0,
@@ -355,7 +355,9 @@ def main():
         import multiprocessing as mp
         pool = mp.get_context("fork").Pool(procs)
     try:
-        if args.only:
+        if args.only == "serve":
+            print(json.dumps(serving(args.serve_requests)), flush=True)
+        elif args.only:
             rec = secondary(args, pool, procs, which=(args.only,))
             if args.copy_calib_gb > 0:
                 rec["copy_calibration_bytes"] = copy_calibration(args.copy_calib_gb)

@@ -209,8 +209,14 @@ def run_loads(server, parent, children, mid, emid, n_req=4000, loads=(0.25, 0.5,
     for rho in loads:
         arr = poisson_arrivals(n_req, rho * cap_rps, rng)
         costs, lat, _, wall = server.replay(arr, ops, a, b, deltaw=dw)
+        # wall = arrivals + the drain of the last batches (about one latency):
+        # the rate served while requests arrive is n over the arrival span
+        # when the drain stays near the latency, i.e. the server kept up
+        span = float(arr[-1])
         res["loads"].append({
             "load": rho, "offered_rps": rho * cap_rps, "achieved_rps": n_req / (wall / 1e6),
+            "arrival_span_ms": span / 1e3, "drain_ms": (wall - span) / 1e3,
+            "kept_up": bool((wall - span) / 1e3 < 2.0 * float(np.percentile(lat, 99)) / 1e3),
             "gcups": cells_all / (wall / 1e6) / 1e9,
             "p50_ms": float(np.percentile(lat, 50)) / 1e3,
             "p99_ms": float(np.percentile(lat, 99)) / 1e3,
