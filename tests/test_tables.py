@@ -109,7 +109,8 @@ def test_linear_hand_set_medians_tcm12():
             assert med[(i << 5) + j] == m, (i, j)
 
 
-@pytest.mark.parametrize("args", [(1, 2, None), (1, 1, None), (1, 1, 3), (2, 3, None), (2, 1, 2)])
+@pytest.mark.parametrize("args", [(1, 2, None), (1, 1, None), (1, 1, 3), (2, 3, None), (2, 1, 2),
+                                  (5, 1, None)])
 def test_product_tables_match_oracle(args):
     """poy5_amd.cost_matrix (Python) and the oracle (C) are separate
     restatements of the same OCaml table builders."""
@@ -151,7 +152,7 @@ def test_3d_tables():
             assert e(a, a, b)[0] == c2c[(a << 5) + b]
 
 
-@pytest.mark.parametrize("tcm", [(1, 1), (1, 2), (2, 1)])
+@pytest.mark.parametrize("tcm", [(1, 1), (1, 2), (2, 1), (5, 1)])
 def test_product_3d_tables_match_oracle(tcm):
     o = oracle.cm3_from_2d(oracle.cm_dna(*tcm))
     p = cost_matrix.three_d_of_two_dim_comb(cost_matrix.of_transformations_and_gaps(*tcm))
@@ -161,3 +162,75 @@ def test_product_3d_tables_match_oracle(tcm):
                     for k in range(1, 32)])
     assert np.array_equal(np.array(o.cost)[idx], p.cost[idx])
     assert np.array_equal(np.array(o.median)[idx], p.median[idx])
+
+
+def test_metric_flag_follows_the_input_not_the_triangle_inequality():
+    """input_is_metric (src/cost_matrix.ml:1117): positive, symmetric and
+    zero on the diagonal.  Substitution 5 / gap 1 breaks the triangle
+    inequality (5 > 1 + 1) and is still metric for POY; an asymmetric or a
+    non-zero diagonal input is not."""
+    assert cost_matrix.of_transformations_and_gaps(5, 1).metric
+    base = [[0 if x == y else (2 if 4 in (x, y) else 2) for y in range(5)] for x in range(5)]
+    base[0][1], base[1][0] = 1, 3
+    assert not cost_matrix.of_list(base).metric
+    base = [[0 if x == y else 2 for y in range(5)] for x in range(5)]
+    base[0][0] = 1
+    assert not cost_matrix.of_list(base).metric
+
+
+def test_non_metric_bitwise_hand_entries():
+    """A non-metric input (A->C 1, C->A 3, every other change 2) fills the
+    sets with the bitwise builder and create_original (src/cost_matrix.ml:
+    721-805, 1180-1183): a state pair (x, y) is priced old(x, x) + old(x, y)
+    through x or old(x, y) + old(y, y) through y, the diagonal entry taken
+    back off; singleton pairs keep their input cost."""
+    base = [[0 if x == y else 2 for y in range(5)] for x in range(5)]
+    base[0][1], base[1][0] = 1, 3
+    m = cost_matrix.of_list(base)
+    k = lambda a, b: (a << 5) + b  # noqa: E731
+    assert m.cost[k(A, C)] == 1 and m.median[k(A, C)] == A | C
+    assert m.cost[k(C, A)] == 3 and m.median[k(C, A)] == A | C
+    assert m.cost[k(A | C, G)] == 2 and m.median[k(A | C, G)] == A | C | G
+    assert m.worst[k(A | C, G)] == 2
+    # {A, C} vs A: the pair (A, A) costs 0 through A; the worst pair is C -> A
+    assert m.cost[k(A | C, A)] == 0 and m.median[k(A | C, A)] == A
+    assert m.worst[k(A | C, A)] == 3
+    base = [[0 if x == y else 2 for y in range(5)] for x in range(5)]
+    base[0][0] = 1  # a cost between equal states
+    m = cost_matrix.of_list(base)
+    assert m.cost[k(A, A)] == 1 and m.median[k(A, A)] == A
+
+
+def test_no_comb_medians_and_tie_breaker():
+    """fill_medians (src/cost_matrix.ml:899-991) without combinations: the
+    tied states of cost(i, k) + cost(k, j) are collected by prepending, so
+    tie_breaker `first` takes the largest tied state and `last` the
+    smallest.  States 1, 2 and gap 3 with 1-2 costing 1 and gaps 2: for
+    (1, 2) both k = 1 and k = 2 cost 1."""
+    base = [[0, 1, 2], [1, 0, 2], [2, 2, 0]]
+    first = cost_matrix.of_list(base, use_comb=False, tie_breaker="first")
+    last = cost_matrix.of_list(base, use_comb=False, tie_breaker="last")
+    assert first.med(1, 2) == 2 and last.med(1, 2) == 1
+    assert first.med(1, 1) == 1 and first.med(3, 3) == 3
+    assert first.c(1, 3) == 2 and first.tail[1] == 2 and first.prepend[2] == 2
+    # level 1 is read as no combinations (of_channel)
+    assert cost_matrix.of_list(base, level=1).med(1, 2) == 2
+
+
+def test_no_comb_all_elements_row():
+    """The all-elements state (4 here): its input costs are zeroed, then
+    cost(i, 4) = min_j cost(i, j) and cost(4, j) = min_i cost(i, j)
+    (src/cost_matrix.ml:363-395); median(i, 4) is the list of i's cheapest
+    column; the column scan for median(4, j) never lowers its running
+    minimum, so it keeps the last state's list: median(4, 1) is the first
+    of mat[3][1], whose ties k = 3 and k = 1 (cost 2 each) were prepended."""
+    base = [[0, 1, 2, 7], [1, 0, 2, 7], [2, 2, 0, 7], [7, 7, 7, 7]]
+    m = cost_matrix.of_list(base, use_comb=False, all_elements=4)
+    assert m.c(1, 4) == 0 and m.c(4, 3) == 0 and m.c(4, 4) == 0
+    assert m.med(1, 4) == 1
+    assert m.med(4, 1) == 3
+
+
+def test_combination_levels_are_not_built():
+    with pytest.raises(NotImplementedError):
+        cost_matrix.of_list([[0] * 5] * 5, level=2)
