@@ -419,3 +419,41 @@ def test_limit_2_and_backtrace_2d_limit_dropins(ctx):
     from poy5_amd._lib import PoyhipError
     with pytest.raises(PoyhipError):
         al.extract_edited_2_limit(a, b, 1, 0, len(a), len(b), 0)
+
+
+def _orc_cm_of(m):
+    """The oracle's table struct filled from a product table set."""
+    o = oracle.OrcCM()
+    o.lcm, o.gap, o.a_sz, o.combinations = 5, 16, 31, 1
+    o.cost_model, o.gap_open = m.cost_model, m.gap_open
+    o.cost[:] = [int(x) for x in m.cost]
+    o.worst[:] = [int(x) for x in m.worst]
+    o.median[:] = [int(x) for x in m.median]
+    o.prepend[:] = [int(x) for x in m.prepend]
+    o.tail[:] = [int(x) for x in m.tail]
+    return o
+
+
+def test_non_metric_tcm_batch_and_dropin(ctx):
+    """An asymmetric, non-metric TCM (A->C 1, C->A 3, gaps 2, other changes
+    2; sets from the bitwise builder with create_original): the linear batch
+    and the drop-in against the oracle on the same tables, so the kernels'
+    orientation of an asymmetric cost matches the reference's."""
+    say("test_non_metric_tcm_batch_and_dropin")
+    from poy5_amd.sequence import Align
+    base = [[0 if x == y else 2 for y in range(5)] for x in range(5)]
+    base[0][1], base[1][0] = 1, 3
+    m = cost_matrix.of_list(base)
+    assert not m.metric
+    ctx.set_cm(m)
+    ocm = _orc_cm_of(m)
+    pairs = workloads.pairs_list(1, 24)
+    costs, outs = run_pairs(ctx, "align_2d", pairs)
+    al = Align(ctx)
+    for k, (a, b) in enumerate(pairs):
+        o1, o2, oc, _ = oracle.align_2(a, b, ocm)
+        assert costs[k] == oc, k
+        assert np.array_equal(outs[k][0], o1) and np.array_equal(outs[k][1], o2), k
+        assert np.array_equal(outs[k][2], oracle.ancestor_2(o1, o2, ocm)), k
+        if k < 6:
+            assert al.cost_2(a, b) == oracle.cost_2(a, b, ocm)[0], k
