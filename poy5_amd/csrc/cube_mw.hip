@@ -326,6 +326,9 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) vo
     auto row0 = [&](int r) { return ring + (long long)(r % NR) * RS; };
     int i = 0, j0 = 0, g = 1, cur = 0;
     bool have = false;  // this group's inner rows are staged in s_rows[cur]
+    // s2 code of the wavefront's row, loaded during the step before (the
+    // first group of a triple has only row j = 0, which needs none)
+    int s2c = 0;
     uint32_t kP3 = 0;
     while (i < L1) {
       __syncthreads();  // the rows of the previous group are in the ring
@@ -359,9 +362,14 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) vo
       int b256 = 0;
       uint32_t kP1 = 0, kP2 = 0;
       if (busy) {
-        b256 = j >= 1 ? 256 * (s2[j] & 31) : 0;
+        b256 = j >= 1 ? 256 * (s2c & 31) : 0;
         kP1 = j >= 1 ? s_pair[(b256 >> 3) + g3].x - kD3S1 + kD3P1 : 0;
         kP2 = (i >= 1 && j >= 1) ? s_pair[(b256 >> 3) + g3].y - kD3S2 + kD3P2 : 0;
+      }
+      // the next group's s2 code, consumed a step later
+      {
+        const int jn = j02 + w;
+        s2c = (i2 < L1 && w < g2 && jn >= 1 && jn < L2) ? (int)s2[jn] : 0;
       }
       // rows read: upper, prev, diag
       const int rU = kind == kRowInner ? (i - 1) * (L2 - 1) + j : 0;
