@@ -45,6 +45,18 @@ class OrcStats(C.Structure):
     ]
 
 
+class OrcNkStats(C.Structure):
+    _fields_ = [
+        ("cells", C.c_int64), ("rounds", C.c_int), ("final_T", C.c_int),
+        ("final_k", C.c_int), ("gap_num", C.c_int),
+    ]
+
+
+# direction flags of the diagonal-store aligner (src/newkkonen.c:41-61)
+NK_ALIGN, NK_DO_DELETE, NK_DO_INSERT = 1, 16, 32
+NK_END_INSERT, NK_END_DELETE, NK_END_DIAG, NK_DO_DIAG, NK_INS_EQ_DEL = 64, 128, 256, 512, 1024
+
+
 def build() -> str:
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
     return _LIB
@@ -105,6 +117,10 @@ def lib():
                                          C.POINTER(C.c_int64), ip, C.POINTER(C.c_int64), ip,
                                          C.POINTER(OrcCM3), ip]
         L.orc_batch_align_3d.restype = C.c_int64
+        for f in ("orc_newkk_align", "orc_newkk_full"):
+            getattr(L, f).argtypes = [u8p, C.c_int, u8p, C.c_int, C.POINTER(OrcCM), C.c_int,
+                                      u8p, u8p, ip, C.POINTER(OrcNkStats)]
+        L.orc_newkk_matrix.argtypes = [u8p, C.c_int, u8p, C.c_int, C.POINTER(OrcCM), ip]
         _lib = L
     return _lib
 
@@ -330,3 +346,81 @@ def median_3(a, b, c, c3):
     out = np.zeros(len(a_) + 1, np.uint8)
     n = lib().orc_median_3(ap, bp, cp, len(a_), C.byref(c3), out.ctypes.data_as(C.POINTER(C.c_uint8)))
     return out[:n].copy()
+
+
+def _newkk(fname, s1, s2, cm, swaped, backtrace):
+    a, ap = _u8(s1)
+    b, bp = _u8(s2)
+    if len(a) > len(b):
+        raise ValueError("pass the shorter one as first")
+    cap = len(a) + len(b)
+    r1 = np.zeros(cap, np.uint8)
+    r2 = np.zeros(cap, np.uint8)
+    n = C.c_int(0)
+    st = OrcNkStats()
+    P = lambda x: x.ctypes.data_as(C.POINTER(C.c_uint8))  # noqa: E731
+    c = getattr(lib(), fname)(ap, len(a), bp, len(b), C.byref(cm), int(swaped),
+                              P(r1) if backtrace else None, P(r2) if backtrace else None,
+                              C.byref(n), C.byref(st))
+    out = dict(cost=c, stats=st)
+    if backtrace:
+        if n.value < 0:
+            raise RuntimeError("the reference backtrace fails on this input")
+        out["r1"] = r1[:n.value].copy()
+        out["r2"] = r2[:n.value].copy()
+    return out
+
+
+def newkk_align(s1, s2, cm, swaped=0, backtrace=True):
+    """newkkonen_CAML_algn_affine + newkkonen_CAML_backtrace_affine (the
+    reference's incremental refill): s1 must not be longer than s2.  Returns
+    dict(cost, stats[, r1, r2])."""
+    return _newkk("orc_newkk_align", s1, s2, cm, swaped, backtrace)
+
+
+def newkk_full(s1, s2, cm, swaped=0, backtrace=True):
+    """The same aligner with every band cell recomputed in every round."""
+    return _newkk("orc_newkk_full", s1, s2, cm, swaped, backtrace)
+
+
+def newkk_matrix(s1, s2, cm):
+    """Full-matrix states: array (len1, len2, 5) of cost, P, Q, ED, CD."""
+    a, ap = _u8(s1)
+    b, bp = _u8(s2)
+    out = np.zeros(len(a) * len(b) * 5, np.int32)
+    lib().orc_newkk_matrix(ap, len(a), bp, len(b), C.byref(cm),
+                           out.ctypes.data_as(C.POINTER(C.c_int)))
+    return out.reshape(len(a), len(b), 5)
+
+
+def newkk_align_2(s1, s2, cm, first_gap=True):
+    """NewkkAlign.align_2 (src/sequence.ml:1883-1913) on the oracle: shorter
+    first with swaped = 1 when exchanged, results swapped back; first_gap
+    False prepends a gap to both and strips the first column.  Returns
+    (s1', s2', cost)."""
+    a = np.asarray(s1, np.uint8)
+    b = np.asarray(s2, np.uint8)
+    if not first_gap:
+        a = np.concatenate([[GAP], a]).astype(np.uint8)
+        b = np.concatenate([[GAP], b]).astype(np.uint8)
+    exchange = len(a) > len(b)
+    x, y = (b, a) if exchange else (a, b)
+    r = newkk_align(x, y, cm, 1 if exchange else 0)
+    p1, p2 = (r["r2"], r["r1"]) if exchange else (r["r1"], r["r2"])
+    if not first_gap:
+        p1, p2 = p1[1:], p2[1:]
+    return p1.copy(), p2.copy(), r["cost"]
+
+
+def newkk_cost_2(s1, s2, cm):
+    """NewkkAlign.cost_2 (src/sequence.ml:1915-1963), affine model: shorter
+    first, an empty sequence replaced by a lone gap."""
+    a = np.asarray(s1, np.uint8)
+    b = np.asarray(s2, np.uint8)
+    if len(a) > len(b):
+        a, b = b, a
+    if len(a) == 0:
+        a = np.array([GAP], np.uint8)
+    if len(b) == 0:
+        b = np.array([GAP], np.uint8)
+    return newkk_align(a, b, cm, 0, backtrace=False)["cost"]

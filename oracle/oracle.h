@@ -202,6 +202,40 @@ int64_t orc_batch_align_3d(int n, const uint8_t *blob, const int64_t *off1, cons
                            const int64_t *off2, const int *len2, const int64_t *off3,
                            const int *len3, const orc_cm3 *c3, int *costs);
 
+/* ---- diagonal-store Ukkonen aligner, affine (src/newkkonen.c; newkk.c) ---- */
+/* direction flags, src/newkkonen.c:41-61 */
+#define NK_START 0
+#define NK_ALIGN 1
+#define NK_DO_DELETE 16
+#define NK_DO_INSERT 32
+#define NK_END_INSERT 64
+#define NK_END_DELETE 128
+#define NK_END_DIAG 256
+#define NK_DO_DIAG 512
+#define NK_INS_EQ_DEL 1024
+
+typedef struct {
+    int64_t cells;  /* cell updates over all rounds */
+    int rounds;
+    int final_T;
+    int final_k;
+    int gap_num;    /* max of the two gap counts of the last cell, last round */
+} orc_nk_stats;
+
+/* newkkonen_CAML_algn_affine (newkkonen.c:1473) and, when r1 is non-NULL,
+ * newkkonen_CAML_backtrace_affine (1847) on the same matrix: cost of s1
+ * (lenX) against s2 (lenY >= lenX); r1/r2 (capacity lenX + lenY) receive the
+ * aligned pair left to right, *rlen its length (-1 where the reference
+ * fails).  The reference's incremental refill, band-only storage. */
+int orc_newkk_align(const uint8_t *s1, int lenX, const uint8_t *s2, int lenY, const orc_cm *cm,
+                    int swaped, uint8_t *r1, uint8_t *r2, int *rlen, orc_nk_stats *st);
+/* The same with every band cell recomputed in every round. */
+int orc_newkk_full(const uint8_t *s1, int lenX, const uint8_t *s2, int lenY, const orc_cm *cm,
+                   int swaped, uint8_t *r1, uint8_t *r2, int *rlen, orc_nk_stats *st);
+/* cost, P, Q, ED, CD of every cell of the full matrix (k = lenX - 1) */
+void orc_newkk_matrix(const uint8_t *s1, int lenX, const uint8_t *s2, int lenY, const orc_cm *cm,
+                      int *out);
+
 #ifdef __cplusplus
 }
 #endif
