@@ -146,6 +146,26 @@ int poyhip_align_affine_3_noalloc(poyhip_ctx *ctx, const uint8_t *si, int li,
                                   uint8_t *resi, uint8_t *resj, int *res_len, int cap,
                                   int *cost);
 
+/* ---- diagonal-store Ukkonen aligner (src/newkkonen.c), affine model ----- */
+/* newkkonen_CAML_algn_affine (src/newkkonen.c:1473) = NewkkAlign's
+ * newkk_cost2_affine (src/sequence.ml:1840): cost of the affine band
+ * aligner with T doubling of s1 against s2, l1 <= l2 (else POYHIP_EINVAL, the
+ * reference's assertion).  swaped only matters to the backtrace, as in the
+ * reference.  Non-affine tables return POYHIP_EINVAL: newkkonen_CAML_algn's
+ * non-affine cells are defective (interior cells take cost 0) and, with no
+ * reset between calls, its refill compares against the previous call's
+ * matrix, so its result is not a function of its inputs.  Tables whose sums
+ * could reach the aligner's infinity (INT_MAX / 2) return POYHIP_EINVAL. */
+int poyhip_newkk_cost_2_affine(poyhip_ctx *ctx, const uint8_t *s1, int l1,
+                               const uint8_t *s2, int l2, int swaped, int *cost);
+/* newkkonen_CAML_algn_affine followed by newkkonen_CAML_backtrace_affine
+ * (src/newkkonen.c:1847) on the same matrix, as NewkkAlign.get_alignment
+ * pairs them (src/sequence.ml:1864-1881): r1, r2 (cap >= l1 + l2 bytes, the
+ * reference's sz1 + sz2) receive the aligned pair, *len its length. */
+int poyhip_newkk_align_2_affine(poyhip_ctx *ctx, const uint8_t *s1, int l1,
+                                const uint8_t *s2, int l2, int swaped, int *cost,
+                                uint8_t *r1, uint8_t *r2, int cap, int *len);
+
 /* algn_CAML_ancestor_2 (src/algn.c:3742): median of an aligned pair with
  * POY's gap rules; cap >= len + 1. */
 int poyhip_ancestor_2(poyhip_ctx *ctx, const uint8_t *a, const uint8_t *b, int len,
@@ -201,6 +221,10 @@ int poyhip_median_3(poyhip_ctx *ctx, const uint8_t *a, const uint8_t *b, const u
 #define POYHIP_OP_ALIGN_2D 3     /* Align.align_2 (linear) + ancestor_2 +    */
                                  /* median_2_with_gaps per pair              */
 #define POYHIP_OP_COST_2D 4      /* Align.cost_2 (linear)                    */
+#define POYHIP_OP_NEWKK_ALIGN 7  /* NewkkAlign.align_2 (affine) + ancestor_2 */
+                                 /* + median_2_with_gaps (DOS.median's       */
+                                 /* use_ukk branch, src/seqCS.ml:1044-1052)  */
+#define POYHIP_OP_NEWKK_COST 8   /* NewkkAlign.cost_2 (affine)               */
 
 typedef struct poyhip_batch poyhip_batch;
 
@@ -216,9 +240,10 @@ int poyhip_batch_create(poyhip_ctx *ctx, int op, int n, const int *len1,
 int poyhip_batch_destroy(poyhip_batch *b);
 /* Bytes of the output block of the batch and the offset of pair k in it.
  * Align ops write four consecutive regions of cap_k bytes per pair (affine:
- * resi, resj, median, medianwg with cap_k = l1+l2+2; linear: s1', s2',
- * ancestor median, median with gaps with cap_k = l1+l2+1).  Each region
- * holds its sequence right-aligned: it starts at cap_k - length. */
+ * resi, resj, median, medianwg with cap_k = l1+l2+2; linear and
+ * diagonal-store: s1', s2', ancestor median, median with gaps with cap_k =
+ * l1+l2+1).  Each region holds its sequence right-aligned: it starts at
+ * cap_k - length. */
 size_t poyhip_batch_out_bytes(const poyhip_batch *b);
 int64_t poyhip_batch_out_offset(const poyhip_batch *b, int k);
 int poyhip_batch_out_cap(const poyhip_batch *b, int k);

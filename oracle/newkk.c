@@ -225,6 +225,10 @@ static int nk_first_time(const nk_mat *m, int oldk, int i, int j) {
 static void nk_expand(nk_mat *m, int newk, int oldk) {
     const int newsize = m->diag_in_use + 2 * (newk - oldk);
     const int64_t end = m->doff[newsize];
+    if (end > m->used) {
+        m->pool = realloc(m->pool, sizeof(nk_cell) * (size_t)end);
+        m->used = end;
+    }
     for (int64_t q = m->doff[m->diag_in_use]; q < end; q++) {
         memset(&m->pool[q], 0xff, sizeof(nk_cell));
     }
@@ -362,8 +366,9 @@ static int nk_fill(nk_mat *m, const uint8_t *s1, int lenX, const uint8_t *s2, in
         o += d < m->bb ? lenX : lenX - ((d - m->bb) / 2 + 1);
     }
     m->doff[m->diag_max] = o;
-    m->pool = malloc(sizeof(nk_cell) * (size_t)(o > 0 ? o : 1));
-    m->used = o;
+    /* band-only storage: the baseband now, more diagonals as k grows */
+    m->used = m->doff[m->bb];
+    m->pool = malloc(sizeof(nk_cell) * (size_t)(m->used > 0 ? m->used : 1));
     /* reset_mat: every cost -1 */
     memset(m->pool, 0xff, sizeof(nk_cell) * (size_t)m->doff[m->bb]);
     *nk_at(m, 0, 0) = nk_origin(go);
@@ -459,8 +464,8 @@ enum { M_TODO, M_DELETE, M_INSERT, M_DIAGONAL, M_ALIGN };
  * direction). */
 typedef const nk_cell *(*nk_get)(void *, int, int);
 
-static int nk_backtrace(nk_get get, void *st, const uint8_t *s1, int lenX, const uint8_t *s2,
-                        int lenY, int swaped, uint8_t *r1, uint8_t *r2) {
+static int nk_backtrace(nk_get get, void *st, int k, const uint8_t *s1, int lenX,
+                        const uint8_t *s2, int lenY, int swaped, uint8_t *r1, uint8_t *r2) {
     const int cap = lenX + lenY;
     uint8_t *t1 = malloc(cap), *t2 = malloc(cap);
     int n = 0;
@@ -471,6 +476,8 @@ static int nk_backtrace(nk_get get, void *st, const uint8_t *s1, int lenX, const
     } else {
         int i = lenX - 1, j = lenY - 1, mode = M_TODO;
         while (i >= 0 && j >= 0) {
+            /* a cell outside the band is get_ukkcost's failure */
+            if (j < i - k || j > i + (lenY - lenX) + k) { n = -1; break; }
             const int dir = get(st, i, j)->dir;
             if (dir == NK_START) {
                 if (i != 0 || j != 0) { n = -1; break; }
@@ -534,13 +541,13 @@ int orc_newkk_align(const uint8_t *s1, int lenX, const uint8_t *s2, int lenY, co
     if (lenX < 1 || lenX > lenY) return -1;
     if (st) memset(st, 0, sizeof *st);
     if (lenX * NK_MUCH_LONGER < lenY) {
-        if (r1) *rlen = nk_backtrace(NULL, NULL, s1, lenX, s2, lenY, swaped, r1, r2);
+        if (r1) *rlen = nk_backtrace(NULL, NULL, 0, s1, lenX, s2, lenY, swaped, r1, r2);
         return nk_trivial_cost(s1, lenX, s2, lenY, cm);
     }
     nk_mat m;
     memset(&m, 0, sizeof m);
     const int cost = nk_fill(&m, s1, lenX, s2, lenY, cm, st);
-    if (r1) *rlen = nk_backtrace(get_mat, &m, s1, lenX, s2, lenY, swaped, r1, r2);
+    if (r1) *rlen = nk_backtrace(get_mat, &m, m.k, s1, lenX, s2, lenY, swaped, r1, r2);
     free(m.pool);
     free(m.doff);
     return cost;
@@ -551,13 +558,13 @@ int orc_newkk_full(const uint8_t *s1, int lenX, const uint8_t *s2, int lenY, con
     if (lenX < 1 || lenX > lenY) return -1;
     if (st) memset(st, 0, sizeof *st);
     if (lenX * NK_MUCH_LONGER < lenY) {
-        if (r1) *rlen = nk_backtrace(NULL, NULL, s1, lenX, s2, lenY, swaped, r1, r2);
+        if (r1) *rlen = nk_backtrace(NULL, NULL, 0, s1, lenX, s2, lenY, swaped, r1, r2);
         return nk_trivial_cost(s1, lenX, s2, lenY, cm);
     }
     nk_dense d;
     memset(&d, 0, sizeof d);
     const int cost = nk_fill_full(&d, s1, lenX, s2, lenY, cm, st);
-    if (r1) *rlen = nk_backtrace(get_dense, &d, s1, lenX, s2, lenY, swaped, r1, r2);
+    if (r1) *rlen = nk_backtrace(get_dense, &d, d.k, s1, lenX, s2, lenY, swaped, r1, r2);
     free(d.a);
     return cost;
 }

@@ -25,6 +25,8 @@ OP_ALIGN_2D = 3
 OP_COST_2D = 4
 OP_ALIGN_3D = 5
 OP_COST_3D = 6
+OP_NEWKK_ALIGN = 7
+OP_NEWKK_COST = 8
 
 
 class PoyhipCM(C.Structure):
@@ -81,6 +83,9 @@ SIGNATURES = {
     "poyhip_align_2d": (C.c_int, [_vp, _u8p, C.c_int, _u8p, C.c_int, C.c_int, C.c_int,
                                   _u8p, _u8p, C.c_int, _ip, _ip]),
     "poyhip_cost_affine_3": (C.c_int, [_vp, _u8p, C.c_int, _u8p, C.c_int, _ip]),
+    "poyhip_newkk_cost_2_affine": (C.c_int, [_vp, _u8p, C.c_int, _u8p, C.c_int, C.c_int, _ip]),
+    "poyhip_newkk_align_2_affine": (C.c_int, [_vp, _u8p, C.c_int, _u8p, C.c_int, C.c_int, _ip,
+                                              _u8p, _u8p, C.c_int, _ip]),
     "poyhip_align_affine_3": (C.c_int, [_vp, _u8p, C.c_int, _u8p, C.c_int, C.c_int,
                                         _u8p, _ip, _u8p, _ip, _u8p, _u8p, _ip, C.c_int, _ip]),
     "poyhip_align_affine_3_noalloc": (C.c_int, [_vp, _u8p, C.c_int, _u8p, C.c_int, C.c_int,

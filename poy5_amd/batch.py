@@ -20,6 +20,8 @@ OPS = {
     "cost_affine": _lib.OP_COST_AFFINE,
     "align_2d": _lib.OP_ALIGN_2D,
     "cost_2d": _lib.OP_COST_2D,
+    "newkk_align": _lib.OP_NEWKK_ALIGN,
+    "newkk_cost": _lib.OP_NEWKK_COST,
 }
 
 
@@ -139,7 +141,7 @@ def run_pairs(ctx: Context, op: str, pairs, deltaw=None, device="cuda", dm_budge
     torch.cuda.synchronize()
     b.run(d_blob, d_off1, d_off2, cost, out, outlen, sync=True)
     costs = cost.cpu().numpy()
-    if b.op in (_lib.OP_COST_AFFINE, _lib.OP_COST_2D):
+    if b.op in (_lib.OP_COST_AFFINE, _lib.OP_COST_2D, _lib.OP_NEWKK_COST):
         b.close()
         return costs, None
     oh = out.cpu().numpy()

@@ -9,9 +9,8 @@ enum MedianOp { kAncestor = 0, kMedianNoGaps = 1, kMedianWithGaps = 2, kWorst = 
 // One thread walks one aligned pair.  out receives the result right-aligned
 // in a region of `cap` bytes; res[0] = length (or cost for kWorst), res[1] =
 // error flag (1: a median entry was 0, algn_ancestor_2's failwith).
-__global__ void k_pair_median(const CmDev* cm, const uint8_t* a, const uint8_t* b, int len,
-                              int op, uint8_t* out, int cap, int* res) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+__device__ void pair_median_serial(const CmDev* cm, const uint8_t* a, const uint8_t* b, int len,
+                                   int op, uint8_t* out, int cap, int* res) {
   const uint8_t* med = cm->median;
   int n = 0, err = 0;
 #define PUT(v) out[cap - 1 - n++] = (uint8_t)(v)
@@ -112,6 +111,12 @@ __global__ void k_pair_median(const CmDev* cm, const uint8_t* a, const uint8_t* 
 #undef PUT
   res[0] = n;
   res[1] = 0;
+}
+
+__global__ void k_pair_median(const CmDev* cm, const uint8_t* a, const uint8_t* b, int len,
+                              int op, uint8_t* out, int cap, int* res) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  pair_median_serial(cm, a, b, len, op, out, cap, res);
 }
 
 }  // namespace poyhip

@@ -26,6 +26,7 @@
 #include "linear.hip"
 #include "linear_mw.hip"
 #include "medians.hip"
+#include "newkk.hip"
 
 using namespace poyhip;
 
@@ -86,6 +87,13 @@ struct poyhip_ctx {
   // largest entry of the 3D cost table, -1 when an entry is negative (the
   // workgroup fill carries 8*value + code keys and needs values below 2^27)
   int cm3_max = -1;
+  // largest entry of the 5-bit cost table, -1 when an entry is negative
+  // (the diagonal-store aligner bounds its sums below its infinity with it)
+  int cm_max = -1;
+  // the diagonal-store fill carries both 16-bit gap counts when a longer
+  // sequence reaches 2^16; POYHIP_NEWKK_COUNTS=both makes it carry both
+  // always (results are the same)
+  bool nk_wide_always = false;
   // device address of this device's copy of `scratch` (poyhip_create looks
   // it up; null when the lookup failed, with the reason in scratch_err)
   uint8_t* scratch = nullptr;
@@ -338,6 +346,112 @@ int run_linear_one(poyhip_ctx* c, const uint8_t* s1, int l1, const uint8_t* s2, 
   return POYHIP_OK;
 }
 
+// ---- diagonal-store aligner (newkkonen.c, affine) -------------------------
+
+// Direction stores above this size run the two-phase plan (rounds without
+// directions, then the final round alone with a store of its exact size).
+constexpr int64_t kNkDirectBytes = (int64_t)8 << 30;
+
+// The aligner's preconditions: an affine table and sums that stay below its
+// infinity (INT_MAX / 2) on every path.
+int nk_check(poyhip_ctx* c, int l1, int l2) {
+  if (!c->has_cm) return set_err(c, POYHIP_ESTATE, "no cost matrix installed");
+  if (c->cm.cost_model != 1)
+    return set_err(c, POYHIP_EINVAL,
+                   "the diagonal-store aligner is built for the affine model only");
+  if (l1 > l2) return set_err(c, POYHIP_EINVAL, "pass the shorter sequence first");
+  const long long go = std::max(c->cm.gap_open, 0);
+  if (c->cm_max < 0 || (c->cm_max + go) * ((long long)l1 + l2 + 2) >= (1ll << 29))
+    return set_err(c, POYHIP_EINVAL, "cost entries too large for the aligner's infinity");
+  return POYHIP_OK;
+}
+
+hipError_t launch_newkk_fill(const NkFillArgs& A, bool dm, bool wide, int grid, hipStream_t st) {
+  if (dm && wide)
+    hipLaunchKernelGGL((k_newkk_fill<true, true>), dim3(grid), dim3(64), 0, st, A);
+  else if (dm)
+    hipLaunchKernelGGL((k_newkk_fill<true, false>), dim3(grid), dim3(64), 0, st, A);
+  else if (wide)
+    hipLaunchKernelGGL((k_newkk_fill<false, true>), dim3(grid), dim3(64), 0, st, A);
+  else
+    hipLaunchKernelGGL((k_newkk_fill<false, false>), dim3(grid), dim3(64), 0, st, A);
+  return hipGetLastError();
+}
+
+// One pair (s1 the rows, l1 <= l2) through the batch kernels: fill, and with
+// `backtrace` the walk into the output block (regions s1', s2').
+int run_newkk_one(poyhip_ctx* c, const uint8_t* s1, int l1, const uint8_t* s2, int l2, int swaped,
+                  bool backtrace, PairResult* out_res, std::vector<uint8_t>* outblk,
+                  int outlen[4]) {
+  PairTable t;
+  int* counter;
+  int rc = stage_pair(c, s1, l1, s2, l2, swaped ? 1 : 0, 0, &t, &counter);
+  if (rc) return rc;
+  const bool wide = l2 >= 65536 || c->nk_wide_always;
+  const int64_t outb = nk_out_bytes(l1, l2);
+  if ((rc = grow(c, c->ws, nk_ws_ints(l2) * sizeof(int)))) return rc;
+  if ((rc = grow(c, c->stab, (n_stripes(l1) + 1) * sizeof(long long)))) return rc;
+  if ((rc = grow(c, c->outblk, kResBytes + outb))) return rc;
+  PairResult* dres = static_cast<PairResult*>(c->outblk.p);
+  int* doutlen = reinterpret_cast<int*>(static_cast<char*>(c->outblk.p) + sizeof(PairResult));
+  uint8_t* dout = static_cast<uint8_t*>(c->outblk.p) + kResBytes;
+  NkFillArgs A;
+  A.pt = t;
+  A.cm = c->d_cm;
+  A.dm = nullptr;
+  A.ws = static_cast<int*>(c->ws.p);
+  A.ws_stride = 0;
+  A.stab = static_cast<long long*>(c->stab.p);
+  A.res = dres;
+  A.counter = counter;
+  const int64_t full_dm = nk_dm_bytes(l1, l2);
+  if (backtrace && full_dm <= kNkDirectBytes) {
+    if ((rc = grow(c, c->dm, full_dm))) return rc;
+    A.dm = static_cast<uint8_t*>(c->dm.p);
+    A.phase = 0;
+    HIPCHK(c, launch_newkk_fill(A, true, wide, 1, c->stream));
+  } else {
+    A.phase = 1;
+    HIPCHK(c, launch_newkk_fill(A, false, wide, 1, c->stream));
+    if (backtrace) {
+      PairResult r1;
+      HIPCHK(c, hipMemcpyAsync(&r1, dres, sizeof r1, hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      if (r1.full != 2) {
+        if ((rc = grow(c, c->dm, nk_dm_exact(l1, l2, r1.newk)))) return rc;
+        A.dm = static_cast<uint8_t*>(c->dm.p);
+        A.phase = 2;
+        HIPCHK(c, hipMemsetAsync(counter, 0, sizeof(int), c->stream));
+        HIPCHK(c, launch_newkk_fill(A, true, wide, 1, c->stream));
+      }
+    }
+  }
+  size_t back = sizeof(PairResult);
+  if (backtrace) {
+    NkBtArgs B;
+    B.pt = t;
+    B.cm = c->d_cm;
+    B.dm = A.dm;
+    B.stab = A.stab;
+    B.res = dres;
+    B.out = dout;
+    B.outlen = doutlen;
+    B.medians = 0;
+    hipLaunchKernelGGL(k_newkk_backtrace, dim3(1), dim3(64), 0, c->stream, B);
+    HIPCHK(c, hipGetLastError());
+    back = kResBytes + outb;
+  }
+  c->h_out.resize(back);
+  HIPCHK(c, hipMemcpyAsync(c->h_out.data(), c->outblk.p, back, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  memcpy(out_res, c->h_out.data(), sizeof(PairResult));
+  if (backtrace) {
+    memcpy(outlen, c->h_out.data() + sizeof(PairResult), 4 * sizeof(int));
+    outblk->assign(c->h_out.begin() + kResBytes, c->h_out.begin() + kResBytes + outb);
+  }
+  return POYHIP_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -368,6 +482,8 @@ int poyhip_create(int device, poyhip_ctx** out) {
   c->cube_mw = !(e && strcmp(e, "wave") == 0);
   e = getenv("POYHIP_LINEAR_FILL");
   c->lin_wg = !(e && strcmp(e, "wave") == 0);
+  e = getenv("POYHIP_NEWKK_COUNTS");
+  c->nk_wide_always = e && strcmp(e, "both") == 0;
   *out = c;
   return POYHIP_OK;
 }
@@ -435,6 +551,13 @@ int poyhip_set_cm(poyhip_ctx* c, const poyhip_cm* cm) {
   for (int k = 0; k < 1024; k++) fits &= cm->cost[k] >= 0 && cm->cost[k] < 32768;
   c->lin_fits = fits;
   c->lin_mw = fits && c->lin_wg;
+  int mx = 0;
+  bool neg = false;
+  for (int k = 0; k < 1024; k++) {
+    neg |= cm->cost[k] < 0;
+    mx = std::max(mx, cm->cost[k]);
+  }
+  c->cm_max = neg ? -1 : mx;
   return POYHIP_OK;
 }
 
@@ -540,6 +663,45 @@ int poyhip_align_2d(poyhip_ctx* c, const uint8_t* s1, int l1, const uint8_t* s2,
   c->last_deltawh = deltawh;
   c->last_s1.assign(s1, s1 + l1);
   c->last_s2.assign(s2, s2 + l2);
+  return POYHIP_OK;
+}
+
+int poyhip_newkk_cost_2_affine(poyhip_ctx* c, const uint8_t* s1, int l1, const uint8_t* s2,
+                               int l2, int swaped, int* cost) {
+  if (!c || !cost || !valid_seq(s1, l1) || !valid_seq(s2, l2)) return POYHIP_EINVAL;
+  int rc = nk_check(c, l1, l2);
+  if (rc) return rc;
+  (void)hipSetDevice(c->device);
+  PairResult r;
+  int outlen[4];
+  rc = run_newkk_one(c, s1, l1, s2, l2, swaped, false, &r, nullptr, outlen);
+  if (rc) return rc;
+  *cost = r.cost;
+  return POYHIP_OK;
+}
+
+int poyhip_newkk_align_2_affine(poyhip_ctx* c, const uint8_t* s1, int l1, const uint8_t* s2,
+                                int l2, int swaped, int* cost, uint8_t* r1, uint8_t* r2, int cap,
+                                int* len) {
+  if (!c || !cost || !r1 || !r2 || !len || !valid_seq(s1, l1) || !valid_seq(s2, l2))
+    return POYHIP_EINVAL;
+  int rc = nk_check(c, l1, l2);
+  if (rc) return rc;
+  if (cap < l1 + l2) return set_err(c, POYHIP_ECAP, "output capacity %d < %d", cap, l1 + l2);
+  (void)hipSetDevice(c->device);
+  PairResult r;
+  int outlen[4];
+  std::vector<uint8_t> blk;
+  rc = run_newkk_one(c, s1, l1, s2, l2, swaped, true, &r, &blk, outlen);
+  if (rc) return rc;
+  const int n = outlen[0];
+  if (n < 0 || n > cap)
+    return set_err(c, POYHIP_EDEVICE, "backtrace failed (length %d)", n);
+  const int bc = (int)nk_out_cap(l1, l2);
+  memcpy(r1, blk.data() + bc - n, n);
+  memcpy(r2, blk.data() + 2 * bc - n, n);
+  *len = n;
+  *cost = r.cost;
   return POYHIP_OK;
 }
 
@@ -843,6 +1005,9 @@ struct poyhip_batch {
   // then direction stores sized to each pair's final band
   bool two_phase = false;
   int64_t dm_budget = 0;
+  // diagonal-store batches: some pair's longer sequence reaches 2^16 (both
+  // 16-bit gap counts carried)
+  bool wide = false;
   std::vector<PairResult> h_res;
   // graph replay
   bool use_graph = false;
@@ -853,7 +1018,11 @@ struct poyhip_batch {
 namespace {
 
 bool op_is_affine(int op) { return op == POYHIP_OP_ALIGN_AFFINE || op == POYHIP_OP_COST_AFFINE; }
-bool op_has_dm(int op) { return op == POYHIP_OP_ALIGN_AFFINE || op == POYHIP_OP_ALIGN_2D; }
+bool op_is_newkk(int op) { return op == POYHIP_OP_NEWKK_ALIGN || op == POYHIP_OP_NEWKK_COST; }
+bool op_is_linear(int op) { return op == POYHIP_OP_ALIGN_2D || op == POYHIP_OP_COST_2D; }
+bool op_has_dm(int op) {
+  return op == POYHIP_OP_ALIGN_AFFINE || op == POYHIP_OP_ALIGN_2D || op == POYHIP_OP_NEWKK_ALIGN;
+}
 
 // Direction-store budget of a batch when the caller gives none: the free
 // device memory less an eighth of the device (at least 8 GiB) left to other
@@ -870,16 +1039,19 @@ size_t auto_dm_budget() {
 int64_t pair_dm_bytes(int op, int la, int lb) {
   if (op == POYHIP_OP_ALIGN_AFFINE) return aff_dm_bytes(la, lb);
   if (op == POYHIP_OP_ALIGN_2D) return lin_dm_bytes(la, lb);
+  if (op == POYHIP_OP_NEWKK_ALIGN) return nk_dm_bytes(la, lb);
   return 0;
 }
 int64_t pair_out_bytes(int op, int la, int lb) {
   if (op == POYHIP_OP_ALIGN_AFFINE) return aff_out_bytes(la, lb);
   if (op == POYHIP_OP_ALIGN_2D) return lin_out_bytes(la, lb);
+  if (op == POYHIP_OP_NEWKK_ALIGN) return nk_out_bytes(la, lb);
   return 0;
 }
 int64_t ws_ints(int op, int lb) {
   if (op == POYHIP_OP_ALIGN_AFFINE) return aff_ws_ints(lb);
   if (op == POYHIP_OP_COST_AFFINE) return affcost_ws_ints(lb);
+  if (op_is_newkk(op)) return nk_ws_ints(lb);
   return lin_ws_slot_ints(lb);
 }
 
@@ -952,7 +1124,7 @@ int batch_enqueue(poyhip_batch* b, const uint8_t* blob, const int64_t* off1, con
     hipLaunchKernelGGL(k_affine_classify, dim3(b->n), dim3(64), 0, st, all, b->d_cls,
                        b->d_counters + 2 * nch);
   }
-  if (!op_is_affine(b->op)) {
+  if (op_is_linear(b->op)) {
     PairTable all = batch_table(b, blob, off1, off2, -1);
     hipLaunchKernelGGL(k_linear_deltawh, dim3((b->n + 255) / 256), dim3(256), 0, st, all, b->d_dw,
                        b->has_deltaw, b->d_dwin);
@@ -992,6 +1164,21 @@ int batch_enqueue(poyhip_batch* b, const uint8_t* blob, const int64_t* off1, con
           A.counter = b->d_counters + 2 * k + 1;
           hipLaunchKernelGGL((k_affine_fill<true, false>), dim3(b->grid), dim3(64), 0, st, A);
         }
+        break;
+      }
+      case POYHIP_OP_NEWKK_ALIGN:
+      case POYHIP_OP_NEWKK_COST: {
+        NkFillArgs A;
+        A.pt = t;
+        A.cm = c->d_cm;
+        A.dm = b->d_dm;
+        A.ws = b->d_ws;
+        A.ws_stride = b->ws_stride;
+        A.stab = b->d_stab;
+        A.res = b->d_res;
+        A.counter = b->d_counters + k;
+        A.phase = 0;
+        HIPCHK(c, launch_newkk_fill(A, b->op == POYHIP_OP_NEWKK_ALIGN, b->wide, b->grid, st));
         break;
       }
       default: {
@@ -1034,6 +1221,17 @@ int batch_enqueue(poyhip_batch* b, const uint8_t* blob, const int64_t* off1, con
       B.out = d_out;
       B.outlen = d_outlen;
       hipLaunchKernelGGL(k_linear_backtrace, dim3(t.n), dim3(64), 0, st, B);
+    } else if (b->op == POYHIP_OP_NEWKK_ALIGN) {
+      NkBtArgs B;
+      B.pt = t;
+      B.cm = c->d_cm;
+      B.dm = b->d_dm;
+      B.stab = b->d_stab;
+      B.res = b->d_res;
+      B.out = d_out;
+      B.outlen = d_outlen;
+      B.medians = 1;
+      hipLaunchKernelGGL(k_newkk_backtrace, dim3(t.n), dim3(64), 0, st, B);
     }
     HIPCHK(c, hipGetLastError());
   }
@@ -1051,7 +1249,7 @@ extern "C" {
 int poyhip_batch_create(poyhip_ctx* c, int op, int n, const int* len1, const int* len2,
                         const int* deltaw, size_t dm_budget, poyhip_batch** out) {
   if (!c || !out || n <= 0 || !len1 || !len2) return POYHIP_EINVAL;
-  if (op < POYHIP_OP_ALIGN_AFFINE || op > POYHIP_OP_COST_2D)
+  if (!(op >= POYHIP_OP_ALIGN_AFFINE && op <= POYHIP_OP_COST_2D) && !op_is_newkk(op))
     return set_err(c, POYHIP_EINVAL, "unknown batch op %d", op);
   if (!c->has_cm) return set_err(c, POYHIP_ESTATE, "no cost matrix installed");
   (void)hipSetDevice(c->device);
@@ -1079,7 +1277,15 @@ int poyhip_batch_create(poyhip_ctx* c, int op, int n, const int* len1, const int
     b->lb[k] = sw ? len1[k] : len2[k];
     maxlb = std::max(maxlb, b->lb[k]);
     work[k] = (double)b->la[k] * (double)b->lb[k];
+    if (op_is_newkk(op)) {
+      const int rc = nk_check(c, b->la[k], b->lb[k]);
+      if (rc) {
+        delete b;
+        return rc;
+      }
+    }
   }
+  b->wide = maxlb >= 65536 || c->nk_wide_always;
   // largest first (LPT order for the persistent kernels)
   b->order.resize(n);
   std::iota(b->order.begin(), b->order.end(), 0);
@@ -1101,7 +1307,7 @@ int poyhip_batch_create(poyhip_ctx* c, int op, int n, const int* len1, const int
   // chunks by direction-store budget
   if (dm_budget == 0) dm_budget = auto_dm_budget();
   b->dm_budget = (int64_t)dm_budget;
-  if (op == POYHIP_OP_ALIGN_2D) {
+  if (op == POYHIP_OP_ALIGN_2D || op == POYHIP_OP_NEWKK_ALIGN) {
     int64_t est = 0;
     for (int k = 0; k < n; k++) est += (pair_dm_bytes(op, b->la[k], b->lb[k]) + 255) & ~(int64_t)255;
     // direction stores sized for the full width would need several chunks
@@ -1210,6 +1416,7 @@ int poyhip_batch_out_cap(const poyhip_batch* b, int k) {
   if (!b || k < 0 || k >= b->n) return -1;
   if (b->op == POYHIP_OP_ALIGN_AFFINE) return b->la[k] + b->lb[k] + 2;
   if (b->op == POYHIP_OP_ALIGN_2D) return (int)lin_out_cap(b->la[k], b->lb[k]);
+  if (b->op == POYHIP_OP_NEWKK_ALIGN) return (int)nk_out_cap(b->la[k], b->lb[k]);
   return 0;
 }
 
@@ -1279,8 +1486,10 @@ int batch_run_two_phase(poyhip_batch* b, const uint8_t* d_blob, const int64_t* d
   hipLaunchKernelGGL(k_zero_ints, dim3((2 * n + 4 + 255) / 256), dim3(256), 0, st, b->d_counters,
                      2 * n + 4);
   PairTable all = batch_table(b, d_blob, d_off1, d_off2, -1);
-  hipLaunchKernelGGL(k_linear_deltawh, dim3((n + 255) / 256), dim3(256), 0, st, all, b->d_dw,
-                     b->has_deltaw, b->d_dwin);
+  const bool nk = b->op == POYHIP_OP_NEWKK_ALIGN;
+  if (!nk)
+    hipLaunchKernelGGL(k_linear_deltawh, dim3((n + 255) / 256), dim3(256), 0, st, all, b->d_dw,
+                       b->has_deltaw, b->d_dwin);
   LinFillArgs A;
   A.pt = all;
   A.cm = c->d_cm;
@@ -1291,8 +1500,19 @@ int batch_run_two_phase(poyhip_batch* b, const uint8_t* d_blob, const int64_t* d
   A.res = b->d_res;
   A.counter = b->d_counters;
   A.phase = 1;
+  NkFillArgs N;
+  N.pt = all;
+  N.cm = c->d_cm;
+  N.dm = b->d_dm;
+  N.ws = b->d_ws;
+  N.ws_stride = b->ws_stride;
+  N.stab = b->d_stab;
+  N.res = b->d_res;
+  N.counter = b->d_counters;
+  N.phase = 1;
   HIPCHK(c, hipEventRecord(ev[1], st));
-  HIPCHK(c, launch_linear_fill(c, A, false, b->grid, st));
+  if (nk) HIPCHK(c, launch_newkk_fill(N, false, b->wide, b->grid, st));
+  else HIPCHK(c, launch_linear_fill(c, A, false, b->grid, st));
   HIPCHK(c, hipEventRecord(ev[2], st));
   b->h_res.resize(n);
   HIPCHK(c, hipMemcpyAsync(b->h_res.data(), b->d_res, sizeof(PairResult) * (size_t)n,
@@ -1307,8 +1527,10 @@ int batch_run_two_phase(poyhip_batch* b, const uint8_t* d_blob, const int64_t* d
   for (int q = 0; q < n; q++) {
     const int k = b->order[q];
     const PairResult& r = b->h_res[k];
-    const int64_t sz =
-        (lin_dm_exact(b->la[k], b->lb[k], r.newk < 0 ? 0 : r.newk, r.full != 0) + 255) & ~(int64_t)255;
+    const int64_t exact =
+        nk ? (r.full == 2 ? 0 : nk_dm_exact(b->la[k], b->lb[k], r.newk))
+           : lin_dm_exact(b->la[k], b->lb[k], r.newk < 0 ? 0 : r.newk, r.full != 0);
+    const int64_t sz = (exact + 255) & ~(int64_t)255;
     if (sz > b->dm_budget)
       return set_err(c, POYHIP_ECAP, "pair %d needs %lld direction bytes", k, (long long)sz);
     if (cur > 0 && cur + sz > b->dm_budget) {
@@ -1327,18 +1549,35 @@ int batch_run_two_phase(poyhip_batch* b, const uint8_t* d_blob, const int64_t* d
     A.pt = t;
     A.counter = b->d_counters + 1 + k;
     A.phase = 2;
+    N.pt = t;
+    N.counter = b->d_counters + 1 + k;
+    N.phase = 2;
     HIPCHK(c, hipEventRecord(ev[1], st));
-    HIPCHK(c, launch_linear_fill(c, A, true, b->grid, st));
+    if (nk) HIPCHK(c, launch_newkk_fill(N, true, b->wide, b->grid, st));
+    else HIPCHK(c, launch_linear_fill(c, A, true, b->grid, st));
     HIPCHK(c, hipEventRecord(ev[2], st));
-    LinBtArgs B;
-    B.pt = t;
-    B.cm = c->d_cm;
-    B.dm = b->d_dm;
-    B.stab = b->d_stab;
-    B.res = b->d_res;
-    B.out = d_out;
-    B.outlen = d_outlen;
-    hipLaunchKernelGGL(k_linear_backtrace, dim3(t.n), dim3(64), 0, st, B);
+    if (nk) {
+      NkBtArgs B;
+      B.pt = t;
+      B.cm = c->d_cm;
+      B.dm = b->d_dm;
+      B.stab = b->d_stab;
+      B.res = b->d_res;
+      B.out = d_out;
+      B.outlen = d_outlen;
+      B.medians = 1;
+      hipLaunchKernelGGL(k_newkk_backtrace, dim3(t.n), dim3(64), 0, st, B);
+    } else {
+      LinBtArgs B;
+      B.pt = t;
+      B.cm = c->d_cm;
+      B.dm = b->d_dm;
+      B.stab = b->d_stab;
+      B.res = b->d_res;
+      B.out = d_out;
+      B.outlen = d_outlen;
+      hipLaunchKernelGGL(k_linear_backtrace, dim3(t.n), dim3(64), 0, st, B);
+    }
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipEventSynchronize(ev[2]));
     float ms = 0.f;

@@ -40,6 +40,32 @@ POYHIP_HD inline int64_t lin_dm_bytes(int La, int Lb) {
 POYHIP_HD inline int64_t lin_out_cap(int La, int Lb) { return (int64_t)La + Lb + 1; }
 POYHIP_HD inline int64_t lin_out_bytes(int La, int Lb) { return 4 * lin_out_cap(La, Lb); }
 
+// ---- diagonal-store aligner (affine newkkonen) -----------------------------
+// per-wavefront workspace: two boundary rows of (P, CD, Q, ED) records and
+// gap counts, 16-byte aligned
+POYHIP_HD inline int64_t nk_ws_ints(int Lb) { return 10 * (((int64_t)Lb + 3) & ~(int64_t)3) + 64; }
+// one direction byte per (step, lane), steps padded to 16; a stripe of the
+// widest band has at most Lb + 63 steps
+POYHIP_HD inline int64_t nk_stripe_bytes(int nsteps) { return (int64_t)((nsteps + 15) & ~15) * 64; }
+POYHIP_HD inline int64_t nk_dm_bytes(int La, int Lb) {
+  return (int64_t)n_stripes(La) * nk_stripe_bytes(Lb + 63);
+}
+// direction bytes of the round with band half-width newk
+POYHIP_HD inline int64_t nk_dm_exact(int La, int Lb, int newk) {
+  const int D = Lb - La;
+  int64_t tot = 0;
+  for (int i0 = 1; i0 <= La - 1; i0 += 64) {
+    const int ilast = i0 + 63 < La - 1 ? i0 + 63 : La - 1;
+    const int jlo = i0 - newk > 0 ? i0 - newk : 0;
+    const int jhi = ilast + D + newk < Lb - 1 ? ilast + D + newk : Lb - 1;
+    tot += nk_stripe_bytes((jhi - jlo + 1) + (ilast - i0));
+  }
+  return tot;
+}
+// outputs: s1', s2' (caller order), ancestor, median with gaps
+POYHIP_HD inline int64_t nk_out_cap(int La, int Lb) { return (int64_t)La + Lb + 1; }
+POYHIP_HD inline int64_t nk_out_bytes(int La, int Lb) { return 4 * nk_out_cap(La, Lb); }
+
 // ---- 3D cube --------------------------------------------------------------
 // rows are cut into segments of 1024 cells (16 per lane)
 POYHIP_HD inline int cube_nseg(int L3) { return (L3 + 1023) / 1024; }

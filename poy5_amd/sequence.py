@@ -319,3 +319,69 @@ class Align:
                                             out.ctypes.data_as(C.POINTER(C.c_uint8)), len(out),
                                             C.byref(n)))
         return out[:n.value].copy()
+
+
+class NewkkAlign:
+    """``Sequence.NewkkAlign`` (src/sequence.ml:1831-2062), affine model:
+    the diagonal-store Ukkonen aligner of src/newkkonen.c on the device."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        self.L = ctx._lib
+
+    # -- externals ---------------------------------------------------------
+    def newkk_cost2_affine(self, s1, s2, swaped: int = 0) -> int:
+        """newkkonen_CAML_algn_affine: s1 must not be longer than s2."""
+        a, ap = _u8(s1)
+        b, bp = _u8(s2)
+        out = C.c_int()
+        self.ctx.chk(self.L.poyhip_newkk_cost_2_affine(self.ctx.h, ap, len(a), bp, len(b),
+                                                       int(swaped), C.byref(out)))
+        return out.value
+
+    def get_alignment(self, s1, s2, swaped: int):
+        """newkkonen_CAML_algn_affine + newkkonen_CAML_backtrace_affine on
+        the same sequences (get_alignment, src/sequence.ml:1864-1881):
+        (s1', s2', cost)."""
+        a, ap = _u8(s1)
+        b, bp = _u8(s2)
+        cap = len(a) + len(b)
+        r1 = np.zeros(cap, np.uint8)
+        r2 = np.zeros(cap, np.uint8)
+        n, cost = C.c_int(), C.c_int()
+        self.ctx.chk(self.L.poyhip_newkk_align_2_affine(
+            self.ctx.h, ap, len(a), bp, len(b), int(swaped), C.byref(cost),
+            r1.ctypes.data_as(C.POINTER(C.c_uint8)), r2.ctypes.data_as(C.POINTER(C.c_uint8)),
+            cap, C.byref(n)))
+        return r1[:n.value].copy(), r2[:n.value].copy(), cost.value
+
+    # -- OCaml wrappers ----------------------------------------------------
+    def align_2(self, s1, s2, first_gap: bool = True):
+        """NewkkAlign.align_2 (src/sequence.ml:1883-1913): shorter first with
+        swaped = 1 when exchanged, results swapped back; first_gap False
+        prepends a gap to both and strips the first column."""
+        def cmp(x, y):
+            if len(x) <= len(y):
+                p1, p2, tc = self.get_alignment(x, y, 0)
+                return p1, p2, tc
+            p1, p2, tc = self.get_alignment(y, x, 1)
+            return p2, p1, tc
+        if first_gap:
+            return cmp(np.asarray(s1, np.uint8), np.asarray(s2, np.uint8))
+        a = np.concatenate([[GAP], np.asarray(s1, np.uint8)]).astype(np.uint8)
+        b = np.concatenate([[GAP], np.asarray(s2, np.uint8)]).astype(np.uint8)
+        p1, p2, tc = cmp(a, b)
+        return p1[1:], p2[1:], tc
+
+    def cost_2(self, s1, s2) -> int:
+        """NewkkAlign.cost_2 (src/sequence.ml:1915-1963), affine model:
+        shorter first, an empty sequence replaced by a lone gap."""
+        a = np.asarray(s1, np.uint8)
+        b = np.asarray(s2, np.uint8)
+        if len(a) > len(b):
+            a, b = b, a
+        if len(a) == 0:
+            a = np.array([GAP], np.uint8)
+        if len(b) == 0:
+            b = np.array([GAP], np.uint8)
+        return self.newkk_cost2_affine(a, b, 0)
