@@ -52,7 +52,8 @@ def parse():
     ap.add_argument("--c1-pairs", type=int, default=1024)
     ap.add_argument("--c4-pairs", type=int, default=2048)
     ap.add_argument("--c5-triples", type=int, default=16384)
-    ap.add_argument("--only", choices=["c1", "c4", "c5", "serve"], default=None,
+    ap.add_argument("--nk-pairs", type=int, default=65536)
+    ap.add_argument("--only", choices=["c1", "c4", "c5", "nk", "serve"], default=None,
                     help="run one secondary configuration or the serving run alone (for "
                          "profiling) and print its record")
     ap.add_argument("--no-tune", action="store_true",
@@ -91,6 +92,15 @@ def cpu_job(job):
         L.orc_batch_align_affine(len(l1), P(blob, C.c_uint8), P(o1, C.c_int64), P(l1, C.c_int),
                                  P(o2, C.c_int64), P(l2, C.c_int), C.byref(cm),
                                  P(costs, C.c_int), C.byref(cells))
+    elif kind == "newkk":
+        _, tcm, blob, o1, l1, o2, l2 = job
+        cm = oracle.cm_dna(*tcm)
+        costs = np.zeros(len(l1), np.int32)
+        cells = C.c_int64()
+        t0 = time.perf_counter()
+        L.orc_batch_newkk(len(l1), P(blob, C.c_uint8), P(o1, C.c_int64), P(l1, C.c_int),
+                          P(o2, C.c_int64), P(l2, C.c_int), C.byref(cm), P(costs, C.c_int),
+                          C.byref(cells))
     elif kind == "linear":
         _, tcm, blob, o1, l1, o2, l2 = job
         cm = oracle.cm_dna(*tcm)
@@ -199,6 +209,68 @@ def _timed(run, steps=2):
 # int32 adds and compares per cell of the reference recurrence
 OPS_LINEAR = 6    # algn_fill_row: 3 candidates x (add + compare)
 OPS_CUBE = 14     # algn_fill_cube: 7 candidates x (add + compare)
+OPS_NEWKK = 35    # newkkonen.c's affine cell: four states' adds, minima and
+                  # tie compares, counted as for algn.c's affine cell
+
+
+def newkk_workload(args, pool, procs):
+    """The diagonal-store affine aligner (src/newkkonen.c) on the config-2
+    pairs: NewkkAlign.align_2 + ancestor_2 + median_2_with_gaps per pair
+    (DOS.median's use_ukk branch), batched, inputs resident.  GCUPS on full
+    cells; roofline of the fill stage on the band cells of every round; the
+    oracle's restatement (the reference's incremental refill) on a sample."""
+    import numpy as np
+    import torch
+    from poy5_amd import cost_matrix, workloads
+    from poy5_amd.batch import Batch
+    from poy5_amd.sequence import Context
+    dev = torch.device("cuda:0")
+    n = args.nk_pairs
+    tcm = (1, 1, 3)
+    print(f"[bench] diagonal-store aligner: {n} config-2 pairs", file=sys.stderr, flush=True)
+    ctx = Context(0)
+    ctx.set_cm(cost_matrix.of_transformations_and_gaps(*tcm))
+    blob, off, ln = workloads.gen_pairs(2, n)
+    l1, l2 = ln[0::2].astype(np.int32), ln[1::2].astype(np.int32)
+    b = Batch(ctx, "newkk_align", l1, l2)
+    d_blob = torch.from_numpy(blob).to(dev)
+    d1 = torch.from_numpy(np.ascontiguousarray(off[0::2])).to(dev)
+    d2 = torch.from_numpy(np.ascontiguousarray(off[1::2])).to(dev)
+    cost, o, ol = b.alloc_outputs(dev)
+    fills = []
+
+    def run():
+        b.run(d_blob, d1, d2, cost, o, ol, sync=True)
+        fills.append(b.fill_ms())
+    dt = _timed(run, steps=2)
+    fill_ms = float(np.mean(fills[1:]))
+    cells = float(np.sum(l1.astype(np.float64) * l2))
+    cb, sl, rounds, _ = b.stats()
+    computed = float(cb.sum())
+    achieved = computed * OPS_NEWKK / (fill_ms / 1e3) / 1e12
+    rec = {"pairs": n, "op": "newkk_align", "tcm": list(tcm), "ms": round(dt * 1e3, 3),
+           "cells_full": cells, "cells_computed": computed, "lane_steps": float(sl.sum()),
+           "gcups": round(cells / dt / 1e9, 2), "chunks": b.chunks(),
+           "mean_rounds": float(rounds.mean()),
+           "roofline": {"bound": "valu", "achieved": round(achieved, 3),
+                        "peak": VALU_PEAK_TOPS, "unit": "TOP/s",
+                        "frac": round(achieved / VALU_PEAK_TOPS, 4),
+                        "ops_per_cell": OPS_NEWKK, "fill_ms": round(fill_ms, 3),
+                        "kernel": "k_newkk_fill<true,false> (fill stage, every doubling round)"}}
+    gpu_costs = cost.cpu().numpy()
+    b.close()
+    del cost, o, ol, d_blob
+    ctx.close()
+    torch.cuda.empty_cache()
+    if pool is not None:
+        stride = 16
+        sample = np.arange(0, n, stride)
+        rec["cpu_baseline"] = cpu_baseline(
+            pool, procs, "newkk", tcm, blob, [off[0::2], off[1::2]], [l1, l2], sample,
+            gpu_costs, f"every {stride}th pair")
+        rec["cpu_baseline"]["sample"] = rec["cpu_baseline"]["sample"].replace(
+            "oracle/oracle.c", "oracle/newkk.c")
+    return rec
 
 
 def secondary(args, pool, procs, which=("c1", "c4", "c5")):
@@ -357,6 +429,8 @@ def main():
     try:
         if args.only == "serve":
             print(json.dumps(serving(args.serve_requests)), flush=True)
+        elif args.only == "nk":
+            print(json.dumps(newkk_workload(args, pool, procs)), flush=True)
         elif args.only:
             rec = secondary(args, pool, procs, which=(args.only,))
             if args.copy_calib_gb > 0:
@@ -517,6 +591,7 @@ def run_bench(args, pool, procs):
     if not args.no_extra:
         print("[bench] secondary configurations", file=sys.stderr, flush=True)
         result["secondary"] = secondary(args, pool, procs)
+        result["secondary"]["newkk_config2"] = newkk_workload(args, pool, procs)
     if not args.no_serve:
         print("[bench] serving: 128-taxon SPR rejoin traffic", file=sys.stderr, flush=True)
         result["serving"] = serving(args.serve_requests)

@@ -120,6 +120,10 @@ def lib():
         for f in ("orc_newkk_align", "orc_newkk_full"):
             getattr(L, f).argtypes = [u8p, C.c_int, u8p, C.c_int, C.POINTER(OrcCM), C.c_int,
                                       u8p, u8p, ip, C.POINTER(OrcNkStats)]
+        L.orc_batch_newkk.argtypes = [C.c_int, u8p, C.POINTER(C.c_int64), ip,
+                                      C.POINTER(C.c_int64), ip, C.POINTER(OrcCM), ip,
+                                      C.POINTER(C.c_int64)]
+        L.orc_batch_newkk.restype = C.c_int64
         L.orc_newkk_matrix.argtypes = [u8p, C.c_int, u8p, C.c_int, C.POINTER(OrcCM), ip]
         _lib = L
     return _lib

@@ -601,3 +601,31 @@ void orc_newkk_matrix(const uint8_t *s1, int lenX, const uint8_t *s2, int lenY, 
         }
     free(d.a);
 }
+
+/* CPU baseline driver: NewkkAlign.align_2 per pair (shorter sequence first,
+ * swaped when exchanged, src/sequence.ml:1885-1888) over a blob; returns the
+ * sum of the costs, costs[k] per pair and *cells_out the cells evaluated. */
+int64_t orc_batch_newkk(int n, const uint8_t *blob, const int64_t *off1, const int *len1,
+                        const int64_t *off2, const int *len2, const orc_cm *cm, int *costs,
+                        int64_t *cells_out) {
+    int64_t sum = 0, cells = 0;
+    for (int k = 0; k < n; k++) {
+        const uint8_t *a = blob + off1[k], *b = blob + off2[k];
+        int la = len1[k], lb = len2[k], sw = 0;
+        if (la > lb) {
+            const uint8_t *t = a; a = b; b = t;
+            const int tl = la; la = lb; lb = tl;
+            sw = 1;
+        }
+        uint8_t *r1 = malloc((size_t)la + lb), *r2 = malloc((size_t)la + lb);
+        int rl = 0;
+        orc_nk_stats st;
+        costs[k] = orc_newkk_align(a, la, b, lb, cm, sw, r1, r2, &rl, &st);
+        sum += costs[k];
+        cells += st.cells;
+        free(r1);
+        free(r2);
+    }
+    if (cells_out) *cells_out = cells;
+    return sum;
+}

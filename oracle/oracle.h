@@ -232,6 +232,11 @@ int orc_newkk_align(const uint8_t *s1, int lenX, const uint8_t *s2, int lenY, co
 /* The same with every band cell recomputed in every round. */
 int orc_newkk_full(const uint8_t *s1, int lenX, const uint8_t *s2, int lenY, const orc_cm *cm,
                    int swaped, uint8_t *r1, uint8_t *r2, int *rlen, orc_nk_stats *st);
+/* NewkkAlign.align_2 per pair of a blob (CPU baseline); returns the sum of
+ * the costs. */
+int64_t orc_batch_newkk(int n, const uint8_t *blob, const int64_t *off1, const int *len1,
+                        const int64_t *off2, const int *len2, const orc_cm *cm, int *costs,
+                        int64_t *cells_out);
 /* cost, P, Q, ED, CD of every cell of the full matrix (k = lenX - 1) */
 void orc_newkk_matrix(const uint8_t *s1, int lenX, const uint8_t *s2, int lenY, const orc_cm *cm,
                       int *out);

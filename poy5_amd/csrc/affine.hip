@@ -361,16 +361,6 @@ __device__ __forceinline__ int max_u16x2(int a, int b) {
                                                             __builtin_bit_cast(rot_u16x2, b)));
 }
 
-typedef unsigned long long lmask;  // one bit per lane of the wavefront
-
-__device__ __forceinline__ lmask ballot(bool b) { return __builtin_amdgcn_ballot_w64(b); }
-
-// acc * 2 + (this lane's bit of m): one add with the lane mask as carry-in.
-__device__ __forceinline__ int shift_in(int acc, lmask m) {
-  int r;
-  asm("v_addc_co_u32_e64 %0, vcc, %1, %1, %2" : "=v"(r) : "v"(acc), "s"(m) : "vcc");
-  return r;
-}
 
 // One round of the gap-free fill on the rotating schedule.  Same outputs as
 // the plain stripe loop of k_affine_fill: cost and gap count of the last

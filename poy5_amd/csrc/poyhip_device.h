@@ -87,6 +87,17 @@ __device__ __forceinline__ int wave_shr1(int v, int lane0) {
   return __builtin_amdgcn_update_dpp(lane0, v, 0x138, 0xF, 0xF, false);
 }
 
+typedef unsigned long long lmask;  // one bit per lane of the wavefront
+
+__device__ __forceinline__ lmask ballot(bool b) { return __builtin_amdgcn_ballot_w64(b); }
+
+// acc * 2 + (this lane's bit of m): one add with the lane mask as carry-in.
+__device__ __forceinline__ int shift_in(int acc, lmask m) {
+  int r;
+  asm("v_addc_co_u32_e64 %0, vcc, %1, %1, %2" : "=v"(r) : "v"(acc), "s"(m) : "vcc");
+  return r;
+}
+
 // Next work item of a persistent wavefront.  Every lane issues the atomic
 // (lane 0 adds 1, the others 0) so the grab sits in uniform control flow and
 // the loop around it keeps a single header: a lane-0-only branch before the
