@@ -430,7 +430,10 @@ def main():
         if args.only == "serve":
             print(json.dumps(serving(args.serve_requests)), flush=True)
         elif args.only == "nk":
-            print(json.dumps(newkk_workload(args, pool, procs)), flush=True)
+            rec = newkk_workload(args, pool, procs)
+            if args.copy_calib_gb > 0:
+                rec["copy_calibration_bytes"] = copy_calibration(args.copy_calib_gb)
+            print(json.dumps(rec), flush=True)
         elif args.only:
             rec = secondary(args, pool, procs, which=(args.only,))
             if args.copy_calib_gb > 0:

@@ -404,6 +404,11 @@ int run_newkk_one(poyhip_ctx* c, const uint8_t* s1, int l1, const uint8_t* s2, i
   A.stab = static_cast<long long*>(c->stab.p);
   A.res = dres;
   A.counter = counter;
+  // the walk follows the fill in the same kernel (outputs s1', s2')
+  A.cm_med = c->d_cm;
+  A.out = backtrace ? dout : nullptr;
+  A.outlen = doutlen;
+  A.medians = 0;
   const int64_t full_dm = nk_dm_bytes(l1, l2);
   if (backtrace && full_dm <= kNkDirectBytes) {
     if ((rc = grow(c, c->dm, full_dm))) return rc;
@@ -417,30 +422,16 @@ int run_newkk_one(poyhip_ctx* c, const uint8_t* s1, int l1, const uint8_t* s2, i
       PairResult r1;
       HIPCHK(c, hipMemcpyAsync(&r1, dres, sizeof r1, hipMemcpyDeviceToHost, c->stream));
       HIPCHK(c, hipStreamSynchronize(c->stream));
-      if (r1.full != 2) {
-        if ((rc = grow(c, c->dm, nk_dm_exact(l1, l2, r1.newk)))) return rc;
-        A.dm = static_cast<uint8_t*>(c->dm.p);
-        A.phase = 2;
-        HIPCHK(c, hipMemsetAsync(counter, 0, sizeof(int), c->stream));
-        HIPCHK(c, launch_newkk_fill(A, true, wide, 1, c->stream));
-      }
+      // the trivial case needs no store; its walk still runs in phase 2
+      if ((rc = grow(c, c->dm, r1.full == 2 ? 0 : nk_dm_exact(l1, l2, r1.newk)))) return rc;
+      A.dm = static_cast<uint8_t*>(c->dm.p);
+      A.phase = 2;
+      HIPCHK(c, hipMemsetAsync(counter, 0, sizeof(int), c->stream));
+      HIPCHK(c, launch_newkk_fill(A, true, wide, 1, c->stream));
     }
   }
   size_t back = sizeof(PairResult);
-  if (backtrace) {
-    NkBtArgs B;
-    B.pt = t;
-    B.cm = c->d_cm;
-    B.dm = A.dm;
-    B.stab = A.stab;
-    B.res = dres;
-    B.out = dout;
-    B.outlen = doutlen;
-    B.medians = 0;
-    hipLaunchKernelGGL(k_newkk_backtrace, dim3(1), dim3(64), 0, c->stream, B);
-    HIPCHK(c, hipGetLastError());
-    back = kResBytes + outb;
-  }
+  if (backtrace) back = kResBytes + outb;
   c->h_out.resize(back);
   HIPCHK(c, hipMemcpyAsync(c->h_out.data(), c->outblk.p, back, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1178,6 +1169,11 @@ int batch_enqueue(poyhip_batch* b, const uint8_t* blob, const int64_t* off1, con
         A.res = b->d_res;
         A.counter = b->d_counters + k;
         A.phase = 0;
+        // each pair's walk, ancestor and median follow its fill
+        A.cm_med = c->d_cm;
+        A.out = b->op == POYHIP_OP_NEWKK_ALIGN ? d_out : nullptr;
+        A.outlen = d_outlen;
+        A.medians = 1;
         HIPCHK(c, launch_newkk_fill(A, b->op == POYHIP_OP_NEWKK_ALIGN, b->wide, b->grid, st));
         break;
       }
@@ -1221,17 +1217,6 @@ int batch_enqueue(poyhip_batch* b, const uint8_t* blob, const int64_t* off1, con
       B.out = d_out;
       B.outlen = d_outlen;
       hipLaunchKernelGGL(k_linear_backtrace, dim3(t.n), dim3(64), 0, st, B);
-    } else if (b->op == POYHIP_OP_NEWKK_ALIGN) {
-      NkBtArgs B;
-      B.pt = t;
-      B.cm = c->d_cm;
-      B.dm = b->d_dm;
-      B.stab = b->d_stab;
-      B.res = b->d_res;
-      B.out = d_out;
-      B.outlen = d_outlen;
-      B.medians = 1;
-      hipLaunchKernelGGL(k_newkk_backtrace, dim3(t.n), dim3(64), 0, st, B);
     }
     HIPCHK(c, hipGetLastError());
   }
@@ -1510,6 +1495,10 @@ int batch_run_two_phase(poyhip_batch* b, const uint8_t* d_blob, const int64_t* d
   N.res = b->d_res;
   N.counter = b->d_counters;
   N.phase = 1;
+  N.cm_med = c->d_cm;
+  N.out = d_out;
+  N.outlen = d_outlen;
+  N.medians = 1;
   HIPCHK(c, hipEventRecord(ev[1], st));
   if (nk) HIPCHK(c, launch_newkk_fill(N, false, b->wide, b->grid, st));
   else HIPCHK(c, launch_linear_fill(c, A, false, b->grid, st));
@@ -1556,18 +1545,7 @@ int batch_run_two_phase(poyhip_batch* b, const uint8_t* d_blob, const int64_t* d
     if (nk) HIPCHK(c, launch_newkk_fill(N, true, b->wide, b->grid, st));
     else HIPCHK(c, launch_linear_fill(c, A, true, b->grid, st));
     HIPCHK(c, hipEventRecord(ev[2], st));
-    if (nk) {
-      NkBtArgs B;
-      B.pt = t;
-      B.cm = c->d_cm;
-      B.dm = b->d_dm;
-      B.stab = b->d_stab;
-      B.res = b->d_res;
-      B.out = d_out;
-      B.outlen = d_outlen;
-      B.medians = 1;
-      hipLaunchKernelGGL(k_newkk_backtrace, dim3(t.n), dim3(64), 0, st, B);
-    } else {
+    if (!nk) {
       LinBtArgs B;
       B.pt = t;
       B.cm = c->d_cm;
