@@ -11,6 +11,10 @@
  * Conventions
  *  - Sequences are byte arrays of 5-bit nucleotide bitsets (A=1 C=2 G=4 T=8
  *    gap=16, IUPAC codes are ORs) and start with POY's leading gap (16).
+ *    The per-pair entry points return POYHIP_EINVAL for a byte of 32 or
+ *    more; batches and the server take device-resident sequences unchecked,
+ *    and such a byte there gives undefined results (the kernels index
+ *    32 x 32 tables with the raw codes).
  *  - Output sequences are written left to right into caller buffers.
  *  - Every function returns POYHIP_OK or a negative POYHIP_E* code and never
  *    aborts; poyhip_last_error() describes the last failure of a context.

@@ -133,7 +133,14 @@ int grow(poyhip_ctx* c, DevBuf& b, size_t n) {
   return POYHIP_OK;
 }
 
-bool valid_seq(const uint8_t* s, int l) { return s != nullptr && l >= 1; }
+// A drop-in's sequence: present, not empty, and every byte a 5-bit code (the
+// kernels index 32 x 32 tables with the raw bytes).
+bool valid_seq(const uint8_t* s, int l) {
+  if (s == nullptr || l < 1) return false;
+  uint8_t any = 0;
+  for (int k = 0; k < l; k++) any |= s[k];
+  return any < 32;
+}
 
 // Offsets of the one-pair descriptor block used by the drop-in calls.
 struct OneDesc {

@@ -191,3 +191,19 @@ def test_newkk_longer_sequence_past_16_bit_counts(ctx):
     r1, r2, c = NewkkAlign(ctx).get_alignment(a, b, 0)
     assert c == want["cost"]
     assert np.array_equal(r1, want["r1"]) and np.array_equal(r2, want["r2"])
+
+
+def test_dropins_refuse_bytes_beyond_5_bit_codes(ctx):
+    # every per-pair entry point checks its sequences before touching the
+    # device: a byte of 32 or more would index past the 32 x 32 tables
+    from poy5_amd.sequence import Align
+    ctx.set_cm(cost_matrix.of_transformations_and_gaps(1, 1, 3))
+    bad = np.array([16, 1, 40, 2], np.uint8)
+    good = np.array([16, 1, 2, 4, 8], np.uint8)
+    with pytest.raises(PoyhipError):
+        NewkkAlign(ctx).newkk_cost2_affine(bad, good)
+    with pytest.raises(PoyhipError):
+        Align(ctx).cost_2_affine(bad, good)
+    ctx.set_cm(cost_matrix.of_transformations_and_gaps(1, 1))
+    with pytest.raises(PoyhipError):
+        Align(ctx).cost_2(good, bad)
