@@ -170,6 +170,25 @@ int poyhip_newkk_align_2_affine(poyhip_ctx *ctx, const uint8_t *s1, int l1,
                                 const uint8_t *s2, int l2, int swaped, int *cost,
                                 uint8_t *r1, uint8_t *r2, int cap, int *len);
 
+/* ---- unions of the implied alignment (src/union.c) ---------------------- */
+/* union_CAML_make (src/union.c:318) = Sequence.Unions.make_union
+ * (src/sequence.ml:2845): union_merge of the unions ua and ub of two
+ * children, along their aligned sequences a, b and aligned median (all of
+ * length lena), into a new union c.  A union is a sequence right-aligned in
+ * a buffer of `cap` bytes plus three int32 arrays of `cap` entries (offset,
+ * union_c1, union_c2: Unions.u, src/sequence.ml:2782), live parts at the
+ * right end; ua and ub pass their sequence buffer and offsets.  c's buffer
+ * and arrays (c_cap entries, Unions.union uses ua_len + ub_len + 2) are
+ * written right-aligned, *c_len receives its length; entries the merge does
+ * not write are 0 (the reference leaves them uninitialised).  The offsets
+ * are 32-bit, the reference's USE_LONG_SEQUENCES layout.  POYHIP_EINVAL
+ * where the reference fails ("infinit loop") or c_cap is too small. */
+int poyhip_union_merge(poyhip_ctx *ctx, const uint8_t *a, const uint8_t *b,
+                       const uint8_t *median, int lena, const uint8_t *ua_s, int ua_len,
+                       int ua_cap, const int32_t *ua_off, const uint8_t *ub_s, int ub_len,
+                       int ub_cap, const int32_t *ub_off, uint8_t *c_s, int c_cap,
+                       int32_t *c_off, int32_t *c_c1, int32_t *c_c2, int *c_len);
+
 /* algn_CAML_ancestor_2 (src/algn.c:3742): median of an aligned pair with
  * POY's gap rules; cap >= len + 1. */
 int poyhip_ancestor_2(poyhip_ctx *ctx, const uint8_t *a, const uint8_t *b, int len,
@@ -268,6 +287,24 @@ int poyhip_batch_stats(poyhip_batch *b, int64_t *cells, int64_t *slots, int *rou
 /* Captures the batch's launches into a HIP graph and replays it on later
  * runs with the same device pointers. */
 int poyhip_batch_use_graph(poyhip_batch *b, int enable);
+
+/* Many union merges in one launch (no reference counterpart), e.g. the
+ * unions of one tree level: job k reads and writes the device blobs d_seq
+ * (bytes) and d_ints (int32) at the offsets it names; d_c_len[k] receives
+ * the new union's length and d_status[k] 0, or POYHIP_EINVAL where the
+ * reference fails or a range of the job lies outside the blobs.  Jobs of
+ * one launch must not write what another reads.  Asynchronous on the
+ * context stream unless `sync` is non-zero. */
+typedef struct {
+  int64_t a, b, median;       /* aligned children and median (d_seq offsets) */
+  int64_t ua_s, ub_s, c_s;    /* union sequence buffers (d_seq offsets) */
+  int64_t ua_off, ub_off;     /* children's offset arrays (d_ints offsets) */
+  int64_t c_off, c_c1, c_c2;  /* new union's arrays (d_ints offsets) */
+  int32_t lena, ua_len, ua_cap, ub_len, ub_cap, c_cap;
+} poyhip_union_job;
+int poyhip_union_merge_batch(poyhip_ctx *ctx, int n, const poyhip_union_job *d_jobs,
+                             uint8_t *d_seq, int64_t seq_bytes, int32_t *d_ints,
+                             int64_t n_ints, int *d_c_len, int *d_status, int sync);
 
 /* Tuning knobs of a context (no reference counterpart: the settings the
  * autotuner, poy5_amd/autotune.py, chooses per length bucket).  A value

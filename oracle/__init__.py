@@ -124,6 +124,10 @@ def lib():
                                       C.POINTER(C.c_int64), ip, C.POINTER(OrcCM), ip,
                                       C.POINTER(C.c_int64)]
         L.orc_batch_newkk.restype = C.c_int64
+        i32p = C.POINTER(C.c_int32)
+        L.orc_union_merge.argtypes = [u8p, u8p, u8p, C.c_int, u8p, C.c_int, C.c_int, i32p,
+                                      u8p, C.c_int, C.c_int, i32p, u8p, C.c_int, i32p, i32p,
+                                      i32p, ip]
         L.orc_newkk_matrix.argtypes = [u8p, C.c_int, u8p, C.c_int, C.POINTER(OrcCM), ip]
         _lib = L
     return _lib
@@ -428,3 +432,60 @@ def newkk_cost_2(s1, s2, cm):
     if len(b) == 0:
         b = np.array([GAP], np.uint8)
     return newkk_align(a, b, cm, 0, backtrace=False)["cost"]
+
+
+class Union:
+    """Sequence.Unions.u (src/sequence.ml:2782): a sequence right-aligned in
+    `cap` symbols and three int32 offset arrays of `cap` entries (offset,
+    union_c1, union_c2)."""
+
+    def __init__(self, seq, length, offset, c1, c2):
+        self.seq = np.asarray(seq, np.uint8)
+        self.length = int(length)
+        self.offset = np.asarray(offset, np.int32)
+        self.c1 = np.asarray(c1, np.int32)
+        self.c2 = np.asarray(c2, np.int32)
+
+    @property
+    def cap(self):
+        return len(self.seq)
+
+    def sequence(self):
+        return self.seq[self.cap - self.length:].copy()
+
+
+def union_leaf(s):
+    """Unions.leaf (src/sequence.ml:2833): offsets 0, union_c1 = union_c2 =
+    0..len-1 right-aligned (the sequence fills its capacity)."""
+    s = np.asarray(s, np.uint8).copy()
+    n = len(s)
+    return Union(s, n, np.zeros(n, np.int32), np.arange(n, dtype=np.int32),
+                 np.arange(n, dtype=np.int32))
+
+
+def union_merge(a, b, median, ua, ub):
+    """Unions.union (src/sequence.ml:2849) = union_CAML_make (src/union.c:318):
+    a, b the aligned children, median their aligned median.  Returns the new
+    Union (capacity len(ua) + len(ub) + 2; entries the merge does not write
+    are 0)."""
+    a_, ap = _u8(a)
+    b_, bp = _u8(b)
+    m_, mp = _u8(median)
+    if not (len(a_) == len(b_) == len(m_)):
+        raise ValueError("aligned sequences of different lengths")
+    cap = ua.length + ub.length + 2
+    cs = np.zeros(cap, np.uint8)
+    co = np.zeros(cap, np.int32)
+    c1 = np.zeros(cap, np.int32)
+    c2 = np.zeros(cap, np.int32)
+    n = C.c_int(0)
+    P8 = lambda x: x.ctypes.data_as(C.POINTER(C.c_uint8))  # noqa: E731
+    P32 = lambda x: x.ctypes.data_as(C.POINTER(C.c_int32))  # noqa: E731
+    uas, uao = np.ascontiguousarray(ua.seq), np.ascontiguousarray(ua.offset)
+    ubs, ubo = np.ascontiguousarray(ub.seq), np.ascontiguousarray(ub.offset)
+    r = lib().orc_union_merge(ap, bp, mp, len(a_), P8(uas), ua.length, ua.cap, P32(uao),
+                              P8(ubs), ub.length, ub.cap, P32(ubo), P8(cs), cap, P32(co),
+                              P32(c1), P32(c2), C.byref(n))
+    if r != 0:
+        raise RuntimeError("union_merge fails on this input")
+    return Union(cs, n.value, co, c1, c2)

@@ -237,6 +237,20 @@ int orc_newkk_full(const uint8_t *s1, int lenX, const uint8_t *s2, int lenY, con
 int64_t orc_batch_newkk(int n, const uint8_t *blob, const int64_t *off1, const int *len1,
                         const int64_t *off2, const int *len2, const orc_cm *cm, int *costs,
                         int64_t *cells_out);
+/* ---- unions (src/union.c; union.c) ---- */
+/* union_merge (union.c:170) behind union_CAML_make (318): merges the unions
+ * ua and ub of two aligned children a, b (length lena, with their aligned
+ * median) into a new union c of capacity c_cap (OCaml's Unions.union uses
+ * len(ua) + len(ub) + 2).  A union is a sequence right-aligned in a buffer
+ * of `cap` symbols with 32-bit offset arrays of `cap` entries; ua/ub need
+ * only their offsets.  c's sequence, offsets and child offsets are written
+ * right-aligned; entries the merge does not write are 0.  Returns 0, or -1
+ * where the reference fails. */
+int orc_union_merge(const uint8_t *a, const uint8_t *b, const uint8_t *median, int lena,
+                    const uint8_t *ua_s, int ua_len, int ua_cap, const int32_t *ua_off,
+                    const uint8_t *ub_s, int ub_len, int ub_cap, const int32_t *ub_off,
+                    uint8_t *c_s, int c_cap, int32_t *c_off, int32_t *c_ca, int32_t *c_cb,
+                    int *c_len);
 /* cost, P, Q, ED, CD of every cell of the full matrix (k = lenX - 1) */
 void orc_newkk_matrix(const uint8_t *s1, int lenX, const uint8_t *s2, int lenY, const orc_cm *cm,
                       int *out);

@@ -66,6 +66,7 @@ _u8p = C.POINTER(C.c_uint8)
 _ip = C.POINTER(C.c_int)
 _i64p = C.POINTER(C.c_int64)
 _vp = C.c_void_p
+_i32p = C.POINTER(C.c_int32)
 SIGNATURES = {
     "poyhip_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
     "poyhip_destroy": (C.c_int, [_vp]),
@@ -83,6 +84,11 @@ SIGNATURES = {
     "poyhip_align_2d": (C.c_int, [_vp, _u8p, C.c_int, _u8p, C.c_int, C.c_int, C.c_int,
                                   _u8p, _u8p, C.c_int, _ip, _ip]),
     "poyhip_cost_affine_3": (C.c_int, [_vp, _u8p, C.c_int, _u8p, C.c_int, _ip]),
+    "poyhip_union_merge": (C.c_int, [_vp, _u8p, _u8p, _u8p, C.c_int, _u8p, C.c_int, C.c_int,
+                                     _i32p, _u8p, C.c_int, C.c_int, _i32p, _u8p, C.c_int,
+                                     _i32p, _i32p, _i32p, _ip]),
+    "poyhip_union_merge_batch": (C.c_int, [_vp, C.c_int, _vp, _vp, C.c_int64, _vp, C.c_int64,
+                                           _vp, _vp, C.c_int]),
     "poyhip_newkk_cost_2_affine": (C.c_int, [_vp, _u8p, C.c_int, _u8p, C.c_int, C.c_int, _ip]),
     "poyhip_newkk_align_2_affine": (C.c_int, [_vp, _u8p, C.c_int, _u8p, C.c_int, C.c_int, _ip,
                                               _u8p, _u8p, C.c_int, _ip]),

@@ -27,6 +27,7 @@
 #include "linear_mw.hip"
 #include "medians.hip"
 #include "newkk.hip"
+#include "unions.hip"
 
 using namespace poyhip;
 
@@ -700,6 +701,91 @@ int poyhip_newkk_align_2_affine(poyhip_ctx* c, const uint8_t* s1, int l1, const 
   memcpy(r2, blk.data() + 2 * bc - n, n);
   *len = n;
   *cost = r.cost;
+  return POYHIP_OK;
+}
+
+int poyhip_union_merge_batch(poyhip_ctx* c, int n, const poyhip_union_job* d_jobs,
+                             uint8_t* d_seq, int64_t seq_bytes, int32_t* d_ints, int64_t n_ints,
+                             int* d_c_len, int* d_status, int sync) {
+  if (!c || n < 0 || (n > 0 && (!d_jobs || !d_seq || !d_ints || !d_c_len || !d_status)))
+    return POYHIP_EINVAL;
+  if (n == 0) return POYHIP_OK;
+  (void)hipSetDevice(c->device);
+  hipLaunchKernelGGL(k_union_merge, dim3((n + 63) / 64), dim3(64), 0, c->stream, d_jobs, n,
+                     d_seq, seq_bytes, d_ints, n_ints, d_c_len, d_status);
+  HIPCHK(c, hipGetLastError());
+  if (sync) HIPCHK(c, hipStreamSynchronize(c->stream));
+  return POYHIP_OK;
+}
+
+int poyhip_union_merge(poyhip_ctx* c, const uint8_t* a, const uint8_t* b, const uint8_t* median,
+                       int lena, const uint8_t* ua_s, int ua_len, int ua_cap,
+                       const int32_t* ua_off, const uint8_t* ub_s, int ub_len, int ub_cap,
+                       const int32_t* ub_off, uint8_t* c_s, int c_cap, int32_t* c_off,
+                       int32_t* c_c1, int32_t* c_c2, int* c_len) {
+  if (!c || !a || !b || !median || !ua_s || !ua_off || !ub_s || !ub_off || !c_s || !c_off ||
+      !c_c1 || !c_c2 || !c_len || lena < 1 || ua_cap < 1 || ub_cap < 1 || c_cap < 1 ||
+      ua_len < 0 || ub_len < 0 || ua_len > ua_cap || ub_len > ub_cap)
+    return POYHIP_EINVAL;
+  (void)hipSetDevice(c->device);
+  // one staging block: job, sequences, then the int arrays (4-byte aligned)
+  const int64_t nseq = 3 * (int64_t)lena + ua_cap + ub_cap + c_cap;
+  const int64_t ints_at = (kDescBytes + nseq + 15) & ~(int64_t)15;
+  const int64_t nints = (int64_t)ua_cap + ub_cap + 3 * (int64_t)c_cap;
+  const int64_t res_at = ints_at + 4 * nints;
+  const size_t need = (size_t)res_at + 16;
+  static_assert(sizeof(poyhip_union_job) <= kDescBytes, "union job in the descriptor block");
+  int rc = grow(c, c->stage, need);
+  if (rc) return rc;
+  c->h_stage.assign(need, 0);
+  uint8_t* h = c->h_stage.data();
+  poyhip_union_job J;
+  memset(&J, 0, sizeof J);
+  int64_t o = 0;
+  auto put = [&](const uint8_t* src, int64_t len) {
+    const int64_t at = o;
+    memcpy(h + kDescBytes + o, src, (size_t)len);
+    o += len;
+    return at;
+  };
+  J.a = put(a, lena);
+  J.b = put(b, lena);
+  J.median = put(median, lena);
+  J.ua_s = put(ua_s, ua_cap);
+  J.ub_s = put(ub_s, ub_cap);
+  J.c_s = o;
+  J.lena = lena;
+  J.ua_len = ua_len;
+  J.ua_cap = ua_cap;
+  J.ub_len = ub_len;
+  J.ub_cap = ub_cap;
+  J.c_cap = c_cap;
+  int32_t* hi = reinterpret_cast<int32_t*>(h + ints_at);
+  memcpy(hi, ua_off, 4 * (size_t)ua_cap);
+  memcpy(hi + ua_cap, ub_off, 4 * (size_t)ub_cap);
+  J.ua_off = 0;
+  J.ub_off = ua_cap;
+  J.c_off = (int64_t)ua_cap + ub_cap;
+  J.c_c1 = J.c_off + c_cap;
+  J.c_c2 = J.c_c1 + c_cap;
+  memcpy(h, &J, sizeof J);
+  uint8_t* d = static_cast<uint8_t*>(c->stage.p);
+  HIPCHK(c, hipMemcpyAsync(d, h, (size_t)res_at, hipMemcpyHostToDevice, c->stream));
+  int* dres = reinterpret_cast<int*>(d + res_at);
+  hipLaunchKernelGGL(k_union_merge, dim3(1), dim3(64), 0, c->stream,
+                     reinterpret_cast<const poyhip_union_job*>(d), 1, d + kDescBytes,
+                     (int64_t)nseq, reinterpret_cast<int32_t*>(d + ints_at), nints, dres, dres + 1);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(h, d, need, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const int* hres = reinterpret_cast<const int*>(h + res_at);
+  if (hres[1] != POYHIP_OK)
+    return set_err(c, POYHIP_EINVAL, "union_merge fails on this input (or c_cap too small)");
+  memcpy(c_s, h + kDescBytes + J.c_s, (size_t)c_cap);
+  memcpy(c_off, hi + J.c_off, 4 * (size_t)c_cap);
+  memcpy(c_c1, hi + J.c_c1, 4 * (size_t)c_cap);
+  memcpy(c_c2, hi + J.c_c2, 4 * (size_t)c_cap);
+  *c_len = hres[0];
   return POYHIP_OK;
 }
 

@@ -1,0 +1,232 @@
+// Union merge of POY's implied-alignment unions (src/union.c: union_merge
+// 170, behind union_CAML_make 318) on gfx950.
+//
+// A merge walks the two children's aligned sequences and their unions from
+// the right, prepending the new union's symbols and writing its offset and
+// child-offset arrays; the steps depend on each other through five cursors,
+// so one lane runs one merge and a batch runs many merges side by side (the
+// unions of one tree level, say).  A union is a sequence right-aligned in
+// `cap` bytes of the byte blob plus three int32 arrays of `cap` entries in
+// the int blob (offset, union_c1, union_c2), their live part at the right
+// end; the child unions need only their sequence and offsets.
+#include "poyhip.h"
+#include "poyhip_device.h"
+
+namespace poyhip {
+
+struct UnCursor {
+  const uint8_t* s;  // sequence buffer (cap bytes)
+  int cap, len;
+  int end, position;  // offset-array and sequence cursors
+};
+
+struct UnOut {
+  uint8_t* s;
+  int32_t *off, *ca, *cb;
+  int cap, len;
+  int end, pca, pcb;
+};
+
+__device__ __forceinline__ int un_get(const UnCursor& u, int pos) {
+  return u.s[u.cap - u.len + pos];
+}
+
+// seq_prepend (src/seq.c:155); false when the capacity is exhausted
+__device__ __forceinline__ bool un_prepend(UnOut& c, int v) {
+  if (c.len >= c.cap) return false;
+  c.len++;
+  c.s[c.cap - c.len] = (uint8_t)v;
+  return true;
+}
+
+// union_prepend_item (src/union.c:79)
+__device__ __forceinline__ bool un_item(UnCursor& src, UnOut& c, int v) {
+  if (src.position >= 0) {
+    v |= un_get(src, src.position);
+    src.position--;
+    src.end--;
+  }
+  return un_prepend(c, v);
+}
+
+// union_prepend_and_fill_items (src/union.c:136): the counts prep..1 of the
+// items prepended since the last fill, their child offsets 0, then the
+// column's symbol with the children's positions
+__device__ __forceinline__ bool un_fill(UnOut& c, int interm, int prep, int apos, int bpos) {
+  if (c.end - prep < 0 || c.pca - prep < 0 || c.pcb - prep < 0) return false;
+  for (int i = prep; i > 0; i--) {
+    c.off[c.end--] = i;
+    c.ca[c.pca--] = 0;
+    c.cb[c.pcb--] = 0;
+  }
+  c.ca[c.pca--] = apos;
+  c.cb[c.pcb--] = bpos;
+  const bool ok = un_prepend(c, interm);
+  c.off[c.end--] = 0;
+  return ok;
+}
+
+// union_merge (src/union.c:170-274).  Returns false where the reference
+// fails (the aligned sequences end while a union still has items) or the
+// new union's capacity is exhausted.
+__device__ bool union_merge_one(const uint8_t* a, const uint8_t* b, const uint8_t* median,
+                                int lena, UnCursor& au, const int32_t* aoff, UnCursor& bu,
+                                const int32_t* boff, UnOut& c) {
+  const int gap = kGap;
+  int items = 0;
+  int i = lena - 1;
+  // begin cursors stay put; a union holds items while begin <= end
+  const int abeg = au.len == 0 ? au.cap - 1 : au.cap - au.len;
+  const int bbeg = bu.len == 0 ? bu.cap - 1 : bu.cap - bu.len;
+  while (abeg <= au.end || bbeg <= bu.end) {
+    if (c.end < 0 || c.pca < 0 || c.pcb < 0) return false;
+    // union_copy_non_homologous (src/union.c:118): the count at each
+    // child's cursor is the number of its items copied across as they are
+    if (au.position > -1) {
+      int k = aoff[au.end];
+      items += k;
+      for (; k > 0; k--)
+        if (!un_item(au, c, gap)) return false;
+    }
+    if (bu.position > -1) {
+      int k = boff[bu.end];
+      items += k;
+      for (; k > 0; k--)
+        if (!un_item(bu, c, gap)) return false;
+    }
+    if (i < 0) return false;  // "infinit loop" in the reference
+    int interm = median[i];
+    if (interm == gap) {
+      if (i == 0) {
+        if (!un_fill(c, gap, items, 0, 0)) return false;
+        au.end--;
+        bu.end--;
+        return true;
+      }
+      bool ok;
+      if (a[i] == gap) ok = un_item(bu, c, gap);
+      else if (b[i] == gap) ok = un_item(au, c, gap);
+      else {
+        // union_prepend_pair (src/union.c:38)
+        int v = gap;
+        if (au.position >= 0) {
+          v |= un_get(au, au.position);
+          au.position--;
+          au.end--;
+        }
+        if (bu.position >= 0) {
+          v |= un_get(bu, bu.position);
+          bu.position--;
+          bu.end--;
+        }
+        ok = un_prepend(c, v);
+      }
+      if (!ok) return false;
+      items += 1;
+    } else {
+      int apos, bpos;
+      // union_move_left (src/union.c:161) as the three cases call it; in
+      // the last case each guard tests the other union's cursor, as written
+      if (a[i] == gap) {
+        apos = 0;
+        bpos = bu.position;
+        if (bpos >= 0) {
+          interm |= un_get(bu, bu.position);
+          bu.position--;
+          bu.end--;
+        }
+      } else if (b[i] == gap) {
+        bpos = 0;
+        apos = au.position;
+        if (apos >= 0) {
+          interm |= un_get(au, au.position);
+          au.position--;
+          au.end--;
+        }
+      } else {
+        bpos = bu.position;
+        apos = au.position;
+        if (bpos >= 0) {
+          interm |= un_get(au, au.position);
+          au.position--;
+          au.end--;
+        }
+        if (apos >= 0) {
+          interm |= un_get(bu, bu.position);
+          bu.position--;
+          bu.end--;
+        }
+      }
+      if (!un_fill(c, interm, items, apos, bpos)) return false;
+      items = 0;
+    }
+    i--;
+  }
+  if (c.len == 0 || c.s[c.cap - c.len] != gap) {
+    if (c.end < 0 || c.pca < 0 || c.pcb < 0 || !un_prepend(c, gap)) return false;
+    c.ca[c.pca] = 0;
+    c.cb[c.pcb] = 0;
+    c.off[c.end] = 0;
+  }
+  return true;
+}
+
+// One lane per merge.  The new union's arrays are cleared first (entries the
+// merge does not write are 0; the reference leaves them uninitialised).
+__device__ __forceinline__ bool un_in(int64_t off, int64_t n, int64_t size) {
+  return off >= 0 && n >= 0 && off <= size && n <= size - off;
+}
+
+__global__ __launch_bounds__(64) void k_union_merge(const poyhip_union_job* jobs, int n,
+                                                    uint8_t* seq, int64_t seq_bytes,
+                                                    int32_t* ints, int64_t n_ints, int* c_len,
+                                                    int* status) {
+  const int k = blockIdx.x * 64 + threadIdx.x;
+  if (k >= n) return;
+  const poyhip_union_job J = jobs[k];
+  // every range the job names lies inside its blob
+  const bool fits = J.lena >= 0 && J.ua_len >= 0 && J.ub_len >= 0 && J.ua_cap >= 1 &&
+                    J.ub_cap >= 1 && J.c_cap >= 1 && J.ua_len <= J.ua_cap &&
+                    J.ub_len <= J.ub_cap && un_in(J.a, J.lena, seq_bytes) &&
+                    un_in(J.b, J.lena, seq_bytes) && un_in(J.median, J.lena, seq_bytes) &&
+                    un_in(J.ua_s, J.ua_cap, seq_bytes) && un_in(J.ub_s, J.ub_cap, seq_bytes) &&
+                    un_in(J.c_s, J.c_cap, seq_bytes) && un_in(J.ua_off, J.ua_cap, n_ints) &&
+                    un_in(J.ub_off, J.ub_cap, n_ints) && un_in(J.c_off, J.c_cap, n_ints) &&
+                    un_in(J.c_c1, J.c_cap, n_ints) && un_in(J.c_c2, J.c_cap, n_ints);
+  if (!fits) {
+    c_len[k] = 0;
+    status[k] = POYHIP_EINVAL;
+    return;
+  }
+  UnOut c;
+  c.s = seq + J.c_s;
+  c.off = ints + J.c_off;
+  c.ca = ints + J.c_c1;
+  c.cb = ints + J.c_c2;
+  c.cap = J.c_cap;
+  c.len = 0;
+  for (int q = 0; q < c.cap; q++) {
+    c.s[q] = 0;
+    c.off[q] = 0;
+    c.ca[q] = 0;
+    c.cb[q] = 0;
+  }
+  c.end = c.pca = c.pcb = c.cap - 1;
+  UnCursor au, bu;
+  au.s = seq + J.ua_s;
+  au.cap = J.ua_cap;
+  au.len = J.ua_len;
+  au.end = au.cap - 1;
+  au.position = au.len - 1;
+  bu.s = seq + J.ub_s;
+  bu.cap = J.ub_cap;
+  bu.len = J.ub_len;
+  bu.end = bu.cap - 1;
+  bu.position = bu.len - 1;
+  const bool ok = union_merge_one(seq + J.a, seq + J.b, seq + J.median, J.lena, au,
+                                  ints + J.ua_off, bu, ints + J.ub_off, c);
+  c_len[k] = c.len;
+  status[k] = ok ? POYHIP_OK : POYHIP_EINVAL;
+}
+
+}  // namespace poyhip

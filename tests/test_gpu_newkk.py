@@ -207,3 +207,23 @@ def test_dropins_refuse_bytes_beyond_5_bit_codes(ctx):
     ctx.set_cm(cost_matrix.of_transformations_and_gaps(1, 1))
     with pytest.raises(PoyhipError):
         Align(ctx).cost_2(good, bad)
+
+
+def test_newkk_config2_sample_and_long_pairs_match_oracle(ctx):
+    # 200 pairs drawn from the config-2 workload (50-5,000 symbols) and two
+    # long pairs of 16k-20k (the long-sequence use of NewkkAlign) through the
+    # batch, against the oracle's incremental refill
+    from poy5_amd import workloads
+    say("newkk config-2 sample and long pairs")
+    ctx.set_cm(cost_matrix.of_transformations_and_gaps(1, 1, 3))
+    ocm = oracle.cm_dna(1, 1, 3)
+    pairs = workloads.pairs_list(2, 65536)[::328][:200]
+    rng = np.random.default_rng(12)
+    for n in (16000, 20000):
+        root = rand_seq(rng, n)
+        pairs.append((mutate(rng, root, 0.01), mutate(rng, root, 0.02)))
+    costs, outs = run_pairs(ctx, "newkk_align", pairs)
+    for k, (a, b) in enumerate(pairs):
+        p1, p2, c = oracle.newkk_align_2(a, b, ocm)
+        assert int(costs[k]) == c, k
+        assert np.array_equal(outs[k][0], p1) and np.array_equal(outs[k][1], p2), k
