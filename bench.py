@@ -53,7 +53,7 @@ def parse():
     ap.add_argument("--c4-pairs", type=int, default=2048)
     ap.add_argument("--c5-triples", type=int, default=16384)
     ap.add_argument("--nk-pairs", type=int, default=65536)
-    ap.add_argument("--only", choices=["c1", "c4", "c5", "nk", "serve"], default=None,
+    ap.add_argument("--only", choices=["c1", "c4", "c5", "nk", "un", "serve"], default=None,
                     help="run one secondary configuration or the serving run alone (for "
                          "profiling) and print its record")
     ap.add_argument("--no-tune", action="store_true",
@@ -211,6 +211,85 @@ OPS_LINEAR = 6    # algn_fill_row: 3 candidates x (add + compare)
 OPS_CUBE = 14     # algn_fill_cube: 7 candidates x (add + compare)
 OPS_NEWKK = 35    # newkkonen.c's affine cell: four states' adds, minima and
                   # tie compares, counted as for algn.c's affine cell
+
+
+def unions_workload(args, pool, procs):
+    """The union merge (src/union.c) of a tree's first level: the unions of
+    16,384 tip pairs of the config-2 workload, their aligned children and
+    median with gaps from the batched align_affine_3 (DOS.median's affine
+    branch), merged in one launch.  Inputs resident; merges per second and
+    the HBM bytes the merges read and write against the HBM roofline; the
+    oracle's restatement on a sample."""
+    import numpy as np
+    import torch
+    from poy5_amd import cost_matrix, unions, workloads
+    from poy5_amd.batch import Batch
+    from poy5_amd.sequence import Context
+    n = 16384
+    print(f"[bench] union merge: {n} tip pairs", file=sys.stderr, flush=True)
+    dev = torch.device("cuda:0")
+    ctx = Context(0)
+    ctx.set_cm(cost_matrix.of_transformations_and_gaps(1, 1, 3))
+    blob, off, ln = workloads.gen_pairs(2, n, seed=7)
+    l1, l2 = ln[0::2].astype(np.int32), ln[1::2].astype(np.int32)
+    b = Batch(ctx, "align_affine", l1, l2)
+    d_blob = torch.from_numpy(blob).to(dev)
+    d1 = torch.from_numpy(np.ascontiguousarray(off[0::2])).to(dev)
+    d2 = torch.from_numpy(np.ascontiguousarray(off[1::2])).to(dev)
+    cost, o, ol = b.alloc_outputs(dev)
+    b.run(d_blob, d1, d2, cost, o, ol, sync=True)
+    oh, lh = o.cpu().numpy(), ol.cpu().numpy()
+    merges = []
+    for k in range(n):
+        ra, rb, _, mwg = b.unpack(oh, lh, k)
+        sa = blob[off[2 * k]: off[2 * k] + ln[2 * k]]
+        sb = blob[off[2 * k + 1]: off[2 * k + 1] + ln[2 * k + 1]]
+        merges.append((ra, rb, mwg, unions.leaf(sa), unions.leaf(sb)))
+    b.close()
+    del cost, o, ol, d_blob
+    lv = unions.Level(ctx, merges, dev)
+    lv.run(sync=True)  # warm-up
+    steps = 5
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        lv.run(sync=True)
+    dt = (time.perf_counter() - t0) / steps
+    res = lv.results()
+    ok = sum(r is not None for r in res)
+    lena = np.array([len(m[0]) for m in merges], np.float64)
+    caps = np.array(lv.caps, np.float64)
+    ulen = np.array([m[3].length + m[4].length for m in merges], np.float64)
+    # bytes a merge must move: the aligned triple, the children's sequences
+    # and offsets, the new union's sequence and three arrays (cleared, then
+    # written)
+    moved = float((3 * lena + 5 * ulen + 13 * caps).sum())
+    rec = {"merges": n, "ok": ok, "ms": round(dt * 1e3, 3),
+           "merges_per_s": round(n / dt, 1),
+           "columns_per_s": round(float(lena.sum()) / dt, 1),
+           "roofline": {"bound": "hbm", "achieved": round(moved / dt / 1e9, 2),
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(moved / dt / 1e9 / HBM_PEAK_GBS, 5),
+                        "kernel": "k_union_merge (one lane per merge: a serial walk)"}}
+    if pool is not None:
+        import oracle
+        sample = list(range(0, n, 64))
+        t0 = time.perf_counter()
+        same = True
+        for k in sample:
+            a, bb, m, ua, ub = merges[k]
+            ou = oracle.union_merge(a, bb, m, oracle.union_leaf(ua.seq), oracle.union_leaf(ub.seq))
+            g = res[k]
+            same &= g is not None and g.length == ou.length and np.array_equal(g.offset, ou.offset) \
+                and np.array_equal(g.c1, ou.c1) and np.array_equal(g.c2, ou.c2)
+        cdt = time.perf_counter() - t0
+        rec["cpu_baseline"] = {"value": round(len(sample) / cdt, 1), "unit": "merges/s",
+                               "cores": 1, "kind": "port",
+                               "sample": f"every 64th merge ({len(sample)}), oracle/union.c "
+                                         f"through ctypes, one process, {cdt:.2f} s",
+                               "results_match_gpu": bool(same)}
+    ctx.close()
+    torch.cuda.empty_cache()
+    return rec
 
 
 def newkk_workload(args, pool, procs):
@@ -429,6 +508,8 @@ def main():
     try:
         if args.only == "serve":
             print(json.dumps(serving(args.serve_requests)), flush=True)
+        elif args.only == "un":
+            print(json.dumps(unions_workload(args, pool, procs)), flush=True)
         elif args.only == "nk":
             rec = newkk_workload(args, pool, procs)
             if args.copy_calib_gb > 0:
@@ -595,6 +676,7 @@ def run_bench(args, pool, procs):
         print("[bench] secondary configurations", file=sys.stderr, flush=True)
         result["secondary"] = secondary(args, pool, procs)
         result["secondary"]["newkk_config2"] = newkk_workload(args, pool, procs)
+        result["secondary"]["union_merge_level"] = unions_workload(args, pool, procs)
     if not args.no_serve:
         print("[bench] serving: 128-taxon SPR rejoin traffic", file=sys.stderr, flush=True)
         result["serving"] = serving(args.serve_requests)

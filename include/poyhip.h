@@ -293,8 +293,11 @@ int poyhip_batch_use_graph(poyhip_batch *b, int enable);
  * (bytes) and d_ints (int32) at the offsets it names; d_c_len[k] receives
  * the new union's length and d_status[k] 0, or POYHIP_EINVAL where the
  * reference fails or a range of the job lies outside the blobs.  Jobs of
- * one launch must not write what another reads.  Asynchronous on the
- * context stream unless `sync` is non-zero. */
+ * one launch must not write what another reads.  One lane walks one merge
+ * and a few consecutive jobs share a wavefront, so a launch runs as long as
+ * its longest merge; jobs ordered by aligned length (longest first) keep
+ * similar walks together.  Asynchronous on the context stream unless `sync`
+ * is non-zero. */
 typedef struct {
   int64_t a, b, median;       /* aligned children and median (d_seq offsets) */
   int64_t ua_s, ub_s, c_s;    /* union sequence buffers (d_seq offsets) */

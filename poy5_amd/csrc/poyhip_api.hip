@@ -95,6 +95,9 @@ struct poyhip_ctx {
   // sequence reaches 2^16; POYHIP_NEWKK_COUNTS=both makes it carry both
   // always (results are the same)
   bool nk_wide_always = false;
+  // union merges per wavefront; 0 picks by level size, POYHIP_UNION_LANES
+  // (a power of two up to 64) fixes it
+  int union_lanes = 0;
   // device address of this device's copy of `scratch` (poyhip_create looks
   // it up; null when the lookup failed, with the reason in scratch_err)
   uint8_t* scratch = nullptr;
@@ -483,6 +486,11 @@ int poyhip_create(int device, poyhip_ctx** out) {
   c->lin_wg = !(e && strcmp(e, "wave") == 0);
   e = getenv("POYHIP_NEWKK_COUNTS");
   c->nk_wide_always = e && strcmp(e, "both") == 0;
+  e = getenv("POYHIP_UNION_LANES");
+  if (e) {
+    const int v = atoi(e);
+    if (v == 1 || v == 2 || v == 4 || v == 8 || v == 16 || v == 32 || v == 64) c->union_lanes = v;
+  }
   *out = c;
   return POYHIP_OK;
 }
@@ -711,8 +719,17 @@ int poyhip_union_merge_batch(poyhip_ctx* c, int n, const poyhip_union_job* d_job
     return POYHIP_EINVAL;
   if (n == 0) return POYHIP_OK;
   (void)hipSetDevice(c->device);
-  hipLaunchKernelGGL(k_union_merge, dim3((n + 63) / 64), dim3(64), 0, c->stream, d_jobs, n,
-                     d_seq, seq_bytes, d_ints, n_ints, d_c_len, d_status);
+  // jobs per wavefront: a level's time is its longest walk, and a walk's
+  // steps are quicker the fewer lanes of its wavefront still issue loads, so
+  // wavefronts stay small (4 merges) unless the level has more than 16 per
+  // CU at that size
+  int lanes = c->union_lanes;
+  if (lanes <= 0) {
+    lanes = 64;
+    while (lanes > 4 && (int64_t)((n + lanes - 1) / lanes) < 16 * (int64_t)c->num_cus) lanes >>= 1;
+  }
+  hipLaunchKernelGGL(k_union_merge, dim3((n + lanes - 1) / lanes), dim3(64), 0, c->stream, d_jobs,
+                     n, lanes, d_seq, seq_bytes, d_ints, n_ints, d_c_len, d_status);
   HIPCHK(c, hipGetLastError());
   if (sync) HIPCHK(c, hipStreamSynchronize(c->stream));
   return POYHIP_OK;
@@ -773,7 +790,7 @@ int poyhip_union_merge(poyhip_ctx* c, const uint8_t* a, const uint8_t* b, const 
   HIPCHK(c, hipMemcpyAsync(d, h, (size_t)res_at, hipMemcpyHostToDevice, c->stream));
   int* dres = reinterpret_cast<int*>(d + res_at);
   hipLaunchKernelGGL(k_union_merge, dim3(1), dim3(64), 0, c->stream,
-                     reinterpret_cast<const poyhip_union_job*>(d), 1, d + kDescBytes,
+                     reinterpret_cast<const poyhip_union_job*>(d), 1, 1, d + kDescBytes,
                      (int64_t)nseq, reinterpret_cast<int32_t*>(d + ints_at), nints, dres, dres + 1);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipMemcpyAsync(h, d, need, hipMemcpyDeviceToHost, c->stream));

@@ -51,25 +51,26 @@ __device__ __forceinline__ bool un_item(UnCursor& src, UnOut& c, int v) {
 
 // union_prepend_and_fill_items (src/union.c:136): the counts prep..1 of the
 // items prepended since the last fill, their child offsets 0, then the
-// column's symbol with the children's positions
+// column's symbol with the children's positions (the new union is cleared
+// before the walk, so writes of 0 only move a cursor)
 __device__ __forceinline__ bool un_fill(UnOut& c, int interm, int prep, int apos, int bpos) {
   if (c.end - prep < 0 || c.pca - prep < 0 || c.pcb - prep < 0) return false;
-  for (int i = prep; i > 0; i--) {
-    c.off[c.end--] = i;
-    c.ca[c.pca--] = 0;
-    c.cb[c.pcb--] = 0;
-  }
-  c.ca[c.pca--] = apos;
-  c.cb[c.pcb--] = bpos;
+  for (int i = prep; i > 0; i--) c.off[c.end--] = i;
+  c.pca -= prep;
+  c.pcb -= prep;
+  if (apos != 0) c.ca[c.pca] = apos;
+  if (bpos != 0) c.cb[c.pcb] = bpos;
+  c.pca--;
+  c.pcb--;
   const bool ok = un_prepend(c, interm);
-  c.off[c.end--] = 0;
+  c.end--;  // offset 0
   return ok;
 }
 
 // union_merge (src/union.c:170-274).  Returns false where the reference
 // fails (the aligned sequences end while a union still has items) or the
 // new union's capacity is exhausted.
-__device__ bool union_merge_one(const uint8_t* a, const uint8_t* b, const uint8_t* median,
+__device__ __forceinline__ bool union_merge_one(const uint8_t* a, const uint8_t* b, const uint8_t* median,
                                 int lena, UnCursor& au, const int32_t* aoff, UnCursor& bu,
                                 const int32_t* boff, UnOut& c) {
   const int gap = kGap;
@@ -164,35 +165,67 @@ __device__ bool union_merge_one(const uint8_t* a, const uint8_t* b, const uint8_
   }
   if (c.len == 0 || c.s[c.cap - c.len] != gap) {
     if (c.end < 0 || c.pca < 0 || c.pcb < 0 || !un_prepend(c, gap)) return false;
-    c.ca[c.pca] = 0;
-    c.cb[c.pcb] = 0;
-    c.off[c.end] = 0;
   }
   return true;
 }
 
-// One lane per merge.  The new union's arrays are cleared first (entries the
-// merge does not write are 0; the reference leaves them uninitialised).
+// every range the job names lies inside its blob
 __device__ __forceinline__ bool un_in(int64_t off, int64_t n, int64_t size) {
   return off >= 0 && n >= 0 && off <= size && n <= size - off;
 }
 
+__device__ __forceinline__ bool un_job_fits(const poyhip_union_job& J, int64_t seq_bytes,
+                                            int64_t n_ints) {
+  return J.lena >= 0 && J.ua_len >= 0 && J.ub_len >= 0 && J.ua_cap >= 1 && J.ub_cap >= 1 &&
+         J.c_cap >= 1 && J.ua_len <= J.ua_cap && J.ub_len <= J.ub_cap &&
+         un_in(J.a, J.lena, seq_bytes) && un_in(J.b, J.lena, seq_bytes) &&
+         un_in(J.median, J.lena, seq_bytes) && un_in(J.ua_s, J.ua_cap, seq_bytes) &&
+         un_in(J.ub_s, J.ub_cap, seq_bytes) && un_in(J.c_s, J.c_cap, seq_bytes) &&
+         un_in(J.ua_off, J.ua_cap, n_ints) && un_in(J.ub_off, J.ub_cap, n_ints) &&
+         un_in(J.c_off, J.c_cap, n_ints) && un_in(J.c_c1, J.c_cap, n_ints) &&
+         un_in(J.c_c2, J.c_cap, n_ints);
+}
+
+// One wavefront (one 64-thread block) takes `lanes` consecutive jobs, one
+// lane per merge; fewer jobs per wavefront spread a level over more SIMDs,
+// which hides more of each serial walk's load latency.  The whole wavefront
+// first clears the new unions of its jobs with coalesced stores (entries the
+// merge does not write are 0; the reference leaves them uninitialised), then
+// each job's lane walks it.
 __global__ __launch_bounds__(64) void k_union_merge(const poyhip_union_job* jobs, int n,
-                                                    uint8_t* seq, int64_t seq_bytes,
+                                                    int lanes, uint8_t* seq, int64_t seq_bytes,
                                                     int32_t* ints, int64_t n_ints, int* c_len,
                                                     int* status) {
-  const int k = blockIdx.x * 64 + threadIdx.x;
-  if (k >= n) return;
-  const poyhip_union_job J = jobs[k];
-  // every range the job names lies inside its blob
-  const bool fits = J.lena >= 0 && J.ua_len >= 0 && J.ub_len >= 0 && J.ua_cap >= 1 &&
-                    J.ub_cap >= 1 && J.c_cap >= 1 && J.ua_len <= J.ua_cap &&
-                    J.ub_len <= J.ub_cap && un_in(J.a, J.lena, seq_bytes) &&
-                    un_in(J.b, J.lena, seq_bytes) && un_in(J.median, J.lena, seq_bytes) &&
-                    un_in(J.ua_s, J.ua_cap, seq_bytes) && un_in(J.ub_s, J.ub_cap, seq_bytes) &&
-                    un_in(J.c_s, J.c_cap, seq_bytes) && un_in(J.ua_off, J.ua_cap, n_ints) &&
-                    un_in(J.ub_off, J.ub_cap, n_ints) && un_in(J.c_off, J.c_cap, n_ints) &&
-                    un_in(J.c_c1, J.c_cap, n_ints) && un_in(J.c_c2, J.c_cap, n_ints);
+  const int lane = threadIdx.x;
+  const int base = blockIdx.x * lanes;
+  const int k = base + lane;
+  const bool mine = lane < lanes && k < n;
+  poyhip_union_job J;
+  bool fits = false;
+  if (mine) {
+    J = jobs[k];
+    fits = un_job_fits(J, seq_bytes, n_ints);
+  }
+  const lmask fit = ballot(fits);
+  const int njobs = n - base < lanes ? n - base : lanes;
+  for (int j = 0; j < njobs; j++) {
+    if (!((fit >> j) & 1)) continue;
+    const poyhip_union_job& Q = jobs[base + j];
+    const int cap = Q.c_cap;
+    uint8_t* cs = seq + Q.c_s;
+    int32_t* co = ints + Q.c_off;
+    int32_t* c1 = ints + Q.c_c1;
+    int32_t* c2 = ints + Q.c_c2;
+    for (int q = lane; q < cap; q += 64) {
+      cs[q] = 0;
+      co[q] = 0;
+      c1[q] = 0;
+      c2[q] = 0;
+    }
+  }
+  // the clearing stores complete before any lane writes its union
+  __syncthreads();
+  if (!mine) return;
   if (!fits) {
     c_len[k] = 0;
     status[k] = POYHIP_EINVAL;
@@ -205,12 +238,6 @@ __global__ __launch_bounds__(64) void k_union_merge(const poyhip_union_job* jobs
   c.cb = ints + J.c_c2;
   c.cap = J.c_cap;
   c.len = 0;
-  for (int q = 0; q < c.cap; q++) {
-    c.s[q] = 0;
-    c.off[q] = 0;
-    c.ca[q] = 0;
-    c.cb[q] = 0;
-  }
   c.end = c.pca = c.pcb = c.cap - 1;
   UnCursor au, bu;
   au.s = seq + J.ua_s;

@@ -82,9 +82,9 @@ def union(ctx: Context, a, b, median, ua: Union, ub: Union) -> Union:
     return Union(cs, n.value, co, c1, c2)
 
 
-def union_level(ctx: Context, merges, device="cuda"):
-    """Many merges in one launch: merges = [(a, b, median, ua, ub), ...].
-    Returns the new unions (None where the reference fails)."""
+def pack_level(merges):
+    """Job descriptors and blobs of many merges: merges = [(a, b, median,
+    ua, ub), ...].  Returns (jobs, seq blob, int blob, new capacities)."""
     seq_parts, int_parts = [], []
     so = io = 0
     jobs = np.zeros(len(merges), JOB_DTYPE)
@@ -117,27 +117,52 @@ def union_level(ctx: Context, merges, device="cuda"):
         j["c_off"] = put_int(np.zeros(cap, np.int32))
         j["c_c1"] = put_int(np.zeros(cap, np.int32))
         j["c_c2"] = put_int(np.zeros(cap, np.int32))
-    dev = torch.device(device)
-    d_seq = torch.from_numpy(np.concatenate(seq_parts)).to(dev)
-    d_int = torch.from_numpy(np.concatenate(int_parts)).to(dev)
-    d_jobs = torch.from_numpy(jobs.view(np.uint8).copy()).to(dev)
-    d_len = torch.zeros(len(merges), dtype=torch.int32, device=dev)
-    d_st = torch.zeros(len(merges), dtype=torch.int32, device=dev)
-    torch.cuda.synchronize()
-    ctx.chk(ctx._lib.poyhip_union_merge_batch(
-        ctx.h, len(merges), C.c_void_p(d_jobs.data_ptr()), C.c_void_p(d_seq.data_ptr()),
-        d_seq.numel(), C.c_void_p(d_int.data_ptr()), d_int.numel(),
-        C.c_void_p(d_len.data_ptr()), C.c_void_p(d_st.data_ptr()), 1))
-    hs, hi = d_seq.cpu().numpy(), d_int.cpu().numpy()
-    hl, hst = d_len.cpu().numpy(), d_st.cpu().numpy()
-    out = []
-    for k in range(len(merges)):
-        if hst[k] != _lib.POYHIP_OK:
-            out.append(None)
-            continue
-        j, cap = jobs[k], caps[k]
-        out.append(Union(hs[j["c_s"]: j["c_s"] + cap].copy(), int(hl[k]),
-                         hi[j["c_off"]: j["c_off"] + cap].copy(),
-                         hi[j["c_c1"]: j["c_c1"] + cap].copy(),
-                         hi[j["c_c2"]: j["c_c2"] + cap].copy()))
-    return out
+    return jobs, np.concatenate(seq_parts), np.concatenate(int_parts), caps
+
+
+class Level:
+    """A packed level resident on the device, run as one launch.  The jobs
+    are laid out longest walk first: a wavefront runs as long as its longest
+    merge, so merges of similar length share wavefronts."""
+
+    def __init__(self, ctx: Context, merges, device="cuda"):
+        self.ctx = ctx
+        self.order = sorted(range(len(merges)), key=lambda k: -len(merges[k][0]))
+        self.jobs, seq, ints, self.caps = pack_level([merges[k] for k in self.order])
+        dev = torch.device(device)
+        self.d_seq = torch.from_numpy(seq).to(dev)
+        self.d_int = torch.from_numpy(ints).to(dev)
+        self.d_jobs = torch.from_numpy(self.jobs.view(np.uint8).copy()).to(dev)
+        self.d_len = torch.zeros(len(merges), dtype=torch.int32, device=dev)
+        self.d_st = torch.zeros(len(merges), dtype=torch.int32, device=dev)
+        torch.cuda.synchronize()
+
+    def run(self, sync=True):
+        self.ctx.chk(self.ctx._lib.poyhip_union_merge_batch(
+            self.ctx.h, len(self.caps), C.c_void_p(self.d_jobs.data_ptr()),
+            C.c_void_p(self.d_seq.data_ptr()), self.d_seq.numel(),
+            C.c_void_p(self.d_int.data_ptr()), self.d_int.numel(),
+            C.c_void_p(self.d_len.data_ptr()), C.c_void_p(self.d_st.data_ptr()),
+            1 if sync else 0))
+
+    def results(self):
+        hs, hi = self.d_seq.cpu().numpy(), self.d_int.cpu().numpy()
+        hl, hst = self.d_len.cpu().numpy(), self.d_st.cpu().numpy()
+        out = [None] * len(self.caps)
+        for k, cap in enumerate(self.caps):
+            if hst[k] != _lib.POYHIP_OK:
+                continue
+            j = self.jobs[k]
+            out[self.order[k]] = (Union(hs[j["c_s"]: j["c_s"] + cap].copy(), int(hl[k]),
+                             hi[j["c_off"]: j["c_off"] + cap].copy(),
+                             hi[j["c_c1"]: j["c_c1"] + cap].copy(),
+                             hi[j["c_c2"]: j["c_c2"] + cap].copy()))
+        return out
+
+
+def union_level(ctx: Context, merges, device="cuda"):
+    """Many merges in one launch: merges = [(a, b, median, ua, ub), ...].
+    Returns the new unions (None where the reference fails)."""
+    lv = Level(ctx, merges, device)
+    lv.run(sync=True)
+    return lv.results()
