@@ -201,6 +201,33 @@ int poyhip_median_2_with_gaps(poyhip_ctx *ctx, const uint8_t *a, const uint8_t *
 /* algn_CAML_worst_2 (src/algn.c:3108) behind Align.max_cost_2. */
 int poyhip_worst_2(poyhip_ctx *ctx, const uint8_t *a, const uint8_t *b, int len,
                    int *cost);
+/* The column map of Align.closest (src/sequence.ml:1180-1236; OCaml, no C
+ * counterpart) for a table with combinations: over the aligned pair
+ * (s1', s2') each column becomes Cost_matrix.Two_D.get_closest(s1'[i],
+ * s2'[i]) (src/cost_matrix.ml:1387), then remove_gaps2 (src/sequence.ml:209)
+ * drops the gaps and prepends one.  `equal` selects the branch for equal
+ * s1 = s2, whose columns after the first lose their gap bit first.  cap >=
+ * len + 1; POYHIP_EINVAL where the reference fails (a 0 symbol). */
+int poyhip_closest_2(poyhip_ctx *ctx, const uint8_t *a, const uint8_t *b, int len,
+                     int equal, uint8_t *out, int cap, int *out_len);
+
+/* Many column maps in one launch (no reference counterpart): for job k,
+ * median_2, median_2_with_gaps or the closest map of the aligned pair at
+ * d_seq + a and d_seq + b (len bytes each) into d_seq + out (cap >= len + 1
+ * bytes, the result right-aligned); d_out_len[k] and d_status[k] (0, or
+ * POYHIP_EINVAL for a failing map or a job outside the blob).  Inputs and
+ * outputs share the blob; jobs must not write what another reads. */
+#define POYHIP_POST_MEDIAN_2 1           /* seq_CAML_median_2_no_gaps   */
+#define POYHIP_POST_MEDIAN_2_WITH_GAPS 2 /* seq_CAML_median_2_with_gaps */
+#define POYHIP_POST_CLOSEST 4            /* poyhip_closest_2, equal = 0  */
+#define POYHIP_POST_CLOSEST_EQ 5         /* poyhip_closest_2, equal = 1  */
+typedef struct {
+  int64_t a, b, out;  /* d_seq offsets */
+  int32_t len, cap;
+} poyhip_post_job;
+int poyhip_pairs_post_batch(poyhip_ctx *ctx, int op, int n, const poyhip_post_job *d_jobs,
+                            uint8_t *d_seq, int64_t seq_bytes, int *d_out_len, int *d_status,
+                            int sync);
 
 /* ---- three sequences ----------------------------------------------------- */
 /* Cost and median entries of POY's struct cm_3d (src/cm.h:253-280) for 5-bit

@@ -129,6 +129,8 @@ def lib():
                                       u8p, C.c_int, C.c_int, i32p, u8p, C.c_int, i32p, i32p,
                                       i32p, ip]
         L.orc_newkk_matrix.argtypes = [u8p, C.c_int, u8p, C.c_int, C.POINTER(OrcCM), ip]
+        L.orc_get_closest.argtypes = [C.POINTER(OrcCM), C.c_int, C.c_int]
+        L.orc_closest_map.argtypes = [u8p, u8p, C.c_int, C.c_int, C.POINTER(OrcCM), u8p]
         _lib = L
     return _lib
 
@@ -489,3 +491,206 @@ def union_merge(a, b, median, ua, ub):
     if r != 0:
         raise RuntimeError("union_merge fails on this input")
     return Union(cs, n.value, co, c1, c2)
+
+
+# ---- Sequence.readjust / DOS.readjust (src/sequence.ml:2097-2146,
+# src/seqCS.ml:776-952): OCaml glue over the functions above, restated here
+# with the same calls and the same order of comparisons ----
+
+def get_closest(cm, a, b):
+    """Cost_matrix.Two_D.get_closest (src/cost_matrix.ml:1387): -1 where the
+    reference fails."""
+    return lib().orc_get_closest(C.byref(cm), int(a), int(b))
+
+
+def closest_map(s1p, s2p, cm, equal=False):
+    """The combine branch of Align.closest (src/sequence.ml:1199-1228): the
+    column map of get_closest over (s1', s2'), then remove_gaps2."""
+    a, ap = _u8(s1p)
+    b, bp = _u8(s2p)
+    if len(a) != len(b):
+        raise ValueError("aligned sequences of different lengths")
+    out = np.zeros(len(a) + 1, np.uint8)
+    n = lib().orc_closest_map(ap, bp, len(a), 1 if equal else 0, C.byref(cm),
+                              out.ctypes.data_as(C.POINTER(C.c_uint8)))
+    if n < 0:
+        raise ValueError("No bits on?")
+    return out[:n].copy()
+
+
+def is_empty(s):
+    """Sequence.is_empty (src/sequence.ml:241): every symbol is the gap."""
+    s = np.asarray(s, np.uint8)
+    return len(s) == 0 or bool(np.all(s == GAP))
+
+
+def select_one(s):
+    """Sequence.select_one (src/sequence.ml:2069) with combinations: the
+    lowest state of each symbol ("Nothing?" for a 0)."""
+    s = np.asarray(s, np.int64)
+    if np.any(s == 0):
+        raise ValueError("Nothing?")
+    return (s & -s).astype(np.uint8)
+
+
+def align_2_any(s1, s2, cm):
+    """Align.align_2 ~first_gap:true (src/sequence.ml:1061): align_affine_3's
+    aligned pair under the affine model, cost_2 + create_edited_2 under the
+    linear one.  Returns (s1', s2', cost)."""
+    s1 = np.asarray(s1, np.uint8)
+    s2 = np.asarray(s2, np.uint8)
+    if cm.cost_model == 1:
+        sw = len(s1) > len(s2)
+        x, y = (s2, s1) if sw else (s1, s2)
+        r = align_affine_3(x, y, cm, int(sw))
+        if sw:
+            return r["resj"], r["resi"], r["cost"]
+        return r["resi"], r["resj"], r["cost"]
+    p1, p2, c, _ = align_2(s1, s2, cm)
+    return p1, p2, c
+
+
+def cost_2_any(s1, s2, cm, deltaw=None):
+    """Align.cost_2 (src/sequence.ml:868): the full affine DP under the
+    affine model, the banded linear cost otherwise."""
+    if cm.cost_model == 1:
+        return cost_affine_3(s1, s2, cm)
+    return cost_2(s1, s2, cm, deltaw)[0]
+
+
+def readjust_algn(s1, s2, cm, use_ukk):
+    """The `algn` of Sequence.readjust (src/sequence.ml:2099-2118): (cost,
+    median) of two sequences."""
+    if cm.cost_model == 1:
+        if use_ukk:
+            p1, p2, c = newkk_align_2(s1, s2, cm)
+            return c, median_2_no_gaps(p1, p2, cm)
+        sw = len(s1) > len(s2)
+        x, y = (s2, s1) if sw else (s1, s2)
+        r = align_affine_3(x, y, cm, int(sw))
+        return r["cost"], r["median"]
+    if use_ukk:
+        raise ValueError("the diagonal-store aligner's linear path is not restated")
+    p1, p2, c = align_2_any(s1, s2, cm)
+    return c, median_2_no_gaps(p1, p2, cm)
+
+
+def closest(s1, s2, cm):
+    """Align.closest (src/sequence.ml:1180-1236), tables with combinations:
+    (sequence, cost)."""
+    s1 = np.asarray(s1, np.uint8)
+    s2 = np.asarray(s2, np.uint8)
+    if is_empty(s2):
+        return s2.copy(), 0
+    if len(s1) == len(s2) and np.array_equal(s1, s2):
+        return closest_map(s1, s2, cm, equal=True), 0
+    p1, p2, _ = align_2_any(s1, s2, cm)
+    r = closest_map(p1, p2, cm)
+    return r, cost_2_any(s1, r, cm)
+
+
+def make_center(a, b, c, cm, use_ukk):
+    """make_center of Sequence.readjust (src/sequence.ml:2120-2136)."""
+    cab, ab = readjust_algn(a, b, cm, use_ukk)
+    cbc, bc = readjust_algn(b, c, cm, use_ukk)
+    cac, ac = readjust_algn(a, c, cm, use_ukk)
+    cabc, _ = readjust_algn(ab, c, cm, use_ukk)
+    cbca, _ = readjust_algn(bc, a, cm, use_ukk)
+    cacb, _ = readjust_algn(ac, b, cm, use_ukk)
+    cabc += cab
+    cbca += cbc
+    cacb += cac
+    if cabc <= cbca:
+        if cabc <= cacb:
+            return closest(c, ab, cm)
+        return closest(b, ac, cm)
+    if cbca < cacb:
+        return closest(a, bc, cm)
+    return closest(b, ac, cm)
+
+
+def seq_readjust(a, b, m, parent, cm, use_ukk=False):
+    """Sequence.readjust (src/sequence.ml:2097-2146): (cost3, cost2, new
+    median, aligned median vs parent x3 as the reference returns it)."""
+    newseqm, _ = make_center(a, b, parent, cm, use_ukk)
+    _, _, c1 = align_2_any(a, newseqm, cm)
+    _, _, c2 = align_2_any(b, newseqm, cm)
+    _, aliedmp, cp = align_2_any(parent, newseqm, cm)
+    cost2 = c1 + c2
+    return cost2 + cp, cost2, newseqm, aliedmp, aliedmp, aliedmp
+
+
+def max_cost_2_any(a, b, cm):
+    """Align.max_cost_2 (src/sequence.ml:652)."""
+    if is_empty(a) or is_empty(b):
+        return 0
+    return worst_2(a, b, cm)
+
+
+def dos_distance(a, b, cm, use_ukk, missing=0):
+    """DOS.distance (src/seqCS.ml:701-775, without USE_VERIFY_COSTS): band
+    deltaw = max(|la - lb|, 8)."""
+    if is_empty(a) or is_empty(b):
+        return missing
+    deltaw = max(abs(len(a) - len(b)), 8)
+    if use_ukk:
+        return newkk_cost_2(a, b, cm)
+    return cost_2_any(a, b, cm, deltaw)
+
+
+def readjust_two(s1, s2, cm, use_ukk):
+    """readjust_algn_two_child (src/seqCS.ml:783-813): (median, cost)."""
+    if cm.cost_model == 1:
+        if use_ukk:
+            p1, p2, c = newkk_align_2(s1, s2, cm)
+            med = median_2_no_gaps(p1, p2, cm)
+        else:
+            c, med = readjust_algn(s1, s2, cm, False)
+        return select_one(med), c
+    if use_ukk:
+        raise ValueError("the diagonal-store aligner's linear path is not restated")
+    p1, p2, c = align_2_any(s1, s2, cm)
+    return select_one(median_2_no_gaps(p1, p2, cm)), c
+
+
+def dos_readjust(ch1, ch2, parent, mine, cm, use_ukk=False):
+    """DOS.readjust (src/seqCS.ml:820-952), ApproxD mode, DNA.  A node is a
+    dict with "seq", "cost2", "cost2_max", "cost3", "sum_cost" (and
+    "aligned" for the result).  Returns (changed, node, cost3, cost2,
+    sum_cost)."""
+    e1, e2, ep = is_empty(ch1["seq"]), is_empty(ch2["seq"]), is_empty(parent["seq"])
+    mc2, msum, mc3 = int(mine["cost2"]), int(mine["sum_cost"]), int(mine["cost3"])
+    sum12 = int(ch1["sum_cost"] + ch2["sum_cost"])
+
+    def node(seq, aligned, c2, c2max, c3, s):
+        return {"seq": seq, "aligned": aligned, "cost2": float(c2), "cost2_max": float(c2max),
+                "cost3": float(c3), "sum_cost": float(s)}
+
+    def differs(seq):
+        return not (len(seq) == len(mine["seq"]) and np.array_equal(seq, mine["seq"]))
+
+    if not e1 and not e2 and not ep:
+        c3, c2, m, al1, al2, mwg = seq_readjust(ch1["seq"], ch2["seq"], mine["seq"],
+                                                parent["seq"], cm, use_ukk)
+        maxc2 = max_cost_2_any(al1, al2, cm)
+        s = c2 + sum12
+        changed = s != msum or c3 != mc3 or c2 != mc2 or differs(m)
+        return changed, node(m, (al1, al2, mwg), c2, maxc2, c3, s), c3, c2, s
+    if (e1 and e2) or (e1 and ep) or (e2 and ep):
+        r = parent if (e1 and e2) else (ch2 if (e1 and ep) else ch1)
+        changed = sum12 != msum or 0 != mc3 or 0 != mc2 or differs(r["seq"])
+        out = dict(r)
+        out["sum_cost"] = float(sum12)
+        return changed, out, 0, 0, sum12
+    if ep:
+        med, c2 = readjust_two(ch1["seq"], ch2["seq"], cm, use_ukk)
+        s = c2 + sum12
+        c3 = c2
+    else:
+        ch = ch1 if e2 else ch2
+        med, _ = readjust_two(ch["seq"], parent["seq"], cm, use_ukk)
+        c2 = dos_distance(ch["seq"], med, cm, use_ukk)
+        s = c2 + sum12
+        c3 = c2 + dos_distance(med, parent["seq"], cm, use_ukk)
+    changed = s != msum or c3 != mc3 or c2 != mc2 or differs(med)
+    return changed, node(med, (med, med, med), c2, 0, c3, s), c3, c2, s

@@ -251,6 +251,14 @@ int orc_union_merge(const uint8_t *a, const uint8_t *b, const uint8_t *median, i
                     const uint8_t *ub_s, int ub_len, int ub_cap, const int32_t *ub_off,
                     uint8_t *c_s, int c_cap, int32_t *c_off, int32_t *c_ca, int32_t *c_cb,
                     int *c_len);
+/* ---- Align.closest's column map (src/sequence.ml:1180; closest.c) ---- */
+/* Cost_matrix.Two_D.get_closest (src/cost_matrix.ml:1387); -1 on failure */
+int orc_get_closest(const orc_cm *cm, int a, int b);
+/* the column map over an aligned pair (or equal sequences, equal != 0) and
+ * remove_gaps2; out holds len + 1 bytes; returns the length or -1 */
+int orc_closest_map(const uint8_t *s1p, const uint8_t *s2p, int len, int equal, const orc_cm *cm,
+                    uint8_t *out);
+
 /* cost, P, Q, ED, CD of every cell of the full matrix (k = lenX - 1) */
 void orc_newkk_matrix(const uint8_t *s1, int lenX, const uint8_t *s2, int lenY, const orc_cm *cm,
                       int *out);

@@ -28,6 +28,11 @@ OP_COST_3D = 6
 OP_NEWKK_ALIGN = 7
 OP_NEWKK_COST = 8
 
+POST_MEDIAN_2 = 1
+POST_MEDIAN_2_WITH_GAPS = 2
+POST_CLOSEST = 4
+POST_CLOSEST_EQ = 5
+
 
 class PoyhipCM(C.Structure):
     _fields_ = [
@@ -101,6 +106,9 @@ SIGNATURES = {
     "poyhip_median_2": (C.c_int, [_vp, _u8p, _u8p, C.c_int, _u8p, C.c_int, _ip]),
     "poyhip_median_2_with_gaps": (C.c_int, [_vp, _u8p, _u8p, C.c_int, _u8p, C.c_int, _ip]),
     "poyhip_worst_2": (C.c_int, [_vp, _u8p, _u8p, C.c_int, _ip]),
+    "poyhip_closest_2": (C.c_int, [_vp, _u8p, _u8p, C.c_int, C.c_int, _u8p, C.c_int, _ip]),
+    "poyhip_pairs_post_batch": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp, C.c_int64, _vp, _vp,
+                                          C.c_int]),
     "poyhip_set_cm3": (C.c_int, [_vp, C.POINTER(PoyhipCM3)]),
     "poyhip_cost_3": (C.c_int, [_vp, _u8p, C.c_int, _u8p, C.c_int, _u8p, C.c_int, C.c_int, _ip]),
     "poyhip_backtrack_3d": (C.c_int, [_vp, _u8p, C.c_int, _u8p, C.c_int, _u8p, C.c_int, _u8p,

@@ -1,10 +1,57 @@
 // Column-wise median lookups on aligned pairs (cm-style table lookups on the
 // device) and the alignment cost recomputation behind Align.max_cost_2.
+#include <climits>
+
+#include "poyhip.h"
 #include "poyhip_device.h"
 
 namespace poyhip {
 
-enum MedianOp { kAncestor = 0, kMedianNoGaps = 1, kMedianWithGaps = 2, kWorst = 3 };
+enum MedianOp {
+  kAncestor = 0,
+  kMedianNoGaps = 1,
+  kMedianWithGaps = 2,
+  kWorst = 3,
+  kClosest = 4,    // Align.closest's column map over an aligned pair
+  kClosestEq = 5   // the same over two equal sequences (gap bits masked)
+};
+
+// Cost_matrix.Two_D.get_closest (src/cost_matrix.ml:1387-1424) for 5-bit
+// sets with combinations: the state of b (its gap bit dropped unless a or b
+// is the gap itself, or the gap when both carry one) cheapest against a,
+// the lowest state winning ties; -1 for an empty set (the reference's
+// "No bits on?" failure)
+__device__ __forceinline__ int closest_state(const int* cost, int a, int b) {
+  int bb;
+  if (a == kGap || b == kGap) bb = b;
+  else if ((a & kGap) && (b & kGap)) bb = kGap;
+  else bb = b & ~kGap;
+  if (bb == 0 || a == 0) return -1;
+  int best = a, bc = INT_MAX;
+#pragma unroll
+  for (int bit = 0; bit < 5; bit++) {
+    const int x = 1 << bit;
+    if (bb & x) {
+      const int c = cost[(a << 5) + x];
+      if (c < bc) {
+        bc = c;
+        best = x;
+      }
+    }
+  }
+  return best;
+}
+
+// one column of Align.closest (src/sequence.ml:1199-1228): the two
+// sequences as the column map sees them (kClosestEq masks the gap bit of
+// every column but the first, as the branch for equal sequences does)
+__device__ __forceinline__ int closest_column(const int* cost, int op, int x, int y, int i) {
+  if (op == kClosestEq && i > 0) {
+    x &= ~kGap;
+    y &= ~kGap;
+  }
+  return closest_state(cost, x, y);
+}
 
 // One thread walks one aligned pair.  out receives the result right-aligned
 // in a region of `cap` bytes; res[0] = length (or cost for kWorst), res[1] =
@@ -45,6 +92,20 @@ __device__ void pair_median_serial(const CmDev* cm, const uint8_t* a, const uint
   if (op == kMedianWithGaps) {
     // seq_get_median_2d_with_gaps (src/seq.c:241)
     for (int i = len - 1; i >= 0; i--) PUT(med[(a[i] << 5) + b[i]]);
+    res[0] = n;
+    res[1] = 0;
+    return;
+  }
+  if (op == kClosest || op == kClosestEq) {
+    // the column map, then remove_gaps2 (src/sequence.ml:209): symbols
+    // equal to the gap dropped and one leading gap prepended
+    for (int i = len - 1; i >= 0; i--) {
+      const int v = closest_column(cm->cost, op, a[i], b[i], i);
+      if (v < 0) { err = 1; break; }
+      if (v != kGap) PUT(v);
+    }
+    if (err) { res[0] = 0; res[1] = 1; return; }
+    PUT(kGap);
     res[0] = n;
     res[1] = 0;
     return;
@@ -117,6 +178,76 @@ __global__ void k_pair_median(const CmDev* cm, const uint8_t* a, const uint8_t* 
                               int op, uint8_t* out, int cap, int* res) {
   if (blockIdx.x != 0 || threadIdx.x != 0) return;
   pair_median_serial(cm, a, b, len, op, out, cap, res);
+}
+
+// Many column maps at once (the medians and closest sequences of a batch of
+// aligned pairs): one wavefront per job, 64 columns a step from the right
+// end; the symbols a map keeps are compacted with a ballot, so each step
+// reads and writes 64 consecutive bytes.  Inputs and outputs share the byte
+// blob `seq`; the result lies right-aligned in the job's `cap` bytes.
+__device__ __forceinline__ bool post_in(int64_t off, int64_t n, int64_t size) {
+  return off >= 0 && n >= 0 && off <= size && n <= size - off;
+}
+
+__global__ __launch_bounds__(256) void k_pairs_post(const CmDev* cm, int op,
+                                                    const poyhip_post_job* jobs, int n,
+                                                    uint8_t* seq, int64_t seq_bytes, int* out_len,
+                                                    int* status) {
+  __shared__ int s_cost[1024];
+  __shared__ uint8_t s_med[1024];
+  for (int q = threadIdx.x; q < 1024; q += 256) {
+    s_cost[q] = cm->cost[q];
+    s_med[q] = cm->median[q];
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (k >= n) return;
+  const poyhip_post_job J = jobs[k];
+  const bool gaps_out = op != kMedianWithGaps;  // a leading gap is prepended
+  if (!(J.len >= 0 && J.cap >= J.len + 1 && post_in(J.a, J.len, seq_bytes) &&
+        post_in(J.b, J.len, seq_bytes) && post_in(J.out, J.cap, seq_bytes))) {
+    if (lane == 0) {
+      out_len[k] = 0;
+      status[k] = POYHIP_EINVAL;
+    }
+    return;
+  }
+  const uint8_t* a = seq + J.a;
+  const uint8_t* b = seq + J.b;
+  // the w-th symbol from the right goes to out[last - w]: indices stay
+  // non-negative (a negated index here was once widened as unsigned)
+  uint8_t* out = seq + J.out;
+  const int last = J.cap - 1;
+  const lmask below = (1ull << lane) - 1;  // lanes holding later columns
+  int w = 0;
+  bool bad = false;
+  for (int base = J.len - 1; base >= 0; base -= 64) {
+    const int i = base - lane;
+    int v = kGap;
+    bool keep = false;
+    if (i >= 0) {
+      const int x = a[i], y = b[i];
+      if (op == kClosest || op == kClosestEq) {
+        v = closest_column(s_cost, op, x, y, i);
+        bad |= v < 0;
+        keep = v > 0 && v != kGap;
+      } else {
+        v = s_med[(x << 5) + y];
+        keep = op == kMedianWithGaps || v != kGap;
+      }
+    }
+    const lmask m = ballot(keep);
+    if (keep) out[last - (w + __popcll(m & below))] = (uint8_t)v;
+    w += __popcll(m);
+  }
+  const bool failed = ballot(bad) != 0;
+  if (gaps_out && lane == 0) out[last - w] = kGap;
+  if (gaps_out) w++;
+  if (lane == 0) {
+    out_len[k] = failed ? 0 : w;
+    status[k] = failed ? POYHIP_EINVAL : POYHIP_OK;
+  }
 }
 
 }  // namespace poyhip

@@ -1065,6 +1065,38 @@ int poyhip_worst_2(poyhip_ctx* c, const uint8_t* a, const uint8_t* b, int len, i
   return pair_median_op(c, a, b, len, kWorst, nullptr, 0, nullptr, cost);
 }
 
+int poyhip_closest_2(poyhip_ctx* c, const uint8_t* a, const uint8_t* b, int len, int equal,
+                     uint8_t* out, int cap, int* out_len) {
+  if (!c || !out || !out_len) return POYHIP_EINVAL;
+  if (c->has_cm && !c->cm.combinations)
+    return set_err(c, POYHIP_EINVAL, "closest: the device map needs a table with combinations");
+  return pair_median_op(c, a, b, len, equal ? kClosestEq : kClosest, out, cap, out_len, nullptr);
+}
+
+int poyhip_pairs_post_batch(poyhip_ctx* c, int op, int n, const poyhip_post_job* d_jobs,
+                            uint8_t* d_seq, int64_t seq_bytes, int* d_out_len, int* d_status,
+                            int sync) {
+  if (!c || n < 0 || (n > 0 && (!d_jobs || !d_seq || !d_out_len || !d_status)))
+    return POYHIP_EINVAL;
+  if (op != POYHIP_POST_MEDIAN_2 && op != POYHIP_POST_MEDIAN_2_WITH_GAPS &&
+      op != POYHIP_POST_CLOSEST && op != POYHIP_POST_CLOSEST_EQ)
+    return set_err(c, POYHIP_EINVAL, "unknown column map %d", op);
+  if (!c->has_cm) return set_err(c, POYHIP_ESTATE, "no cost matrix installed");
+  if ((op == POYHIP_POST_CLOSEST || op == POYHIP_POST_CLOSEST_EQ) && !c->cm.combinations)
+    return set_err(c, POYHIP_EINVAL, "closest: the device map needs a table with combinations");
+  if (n == 0) return POYHIP_OK;
+  static_assert(POYHIP_POST_MEDIAN_2 == kMedianNoGaps && POYHIP_POST_MEDIAN_2_WITH_GAPS ==
+                    kMedianWithGaps && POYHIP_POST_CLOSEST == kClosest &&
+                    POYHIP_POST_CLOSEST_EQ == kClosestEq,
+                "column-map codes");
+  (void)hipSetDevice(c->device);
+  hipLaunchKernelGGL(k_pairs_post, dim3((n + 3) / 4), dim3(256), 0, c->stream, c->d_cm, op,
+                     d_jobs, n, d_seq, seq_bytes, d_out_len, d_status);
+  HIPCHK(c, hipGetLastError());
+  if (sync) HIPCHK(c, hipStreamSynchronize(c->stream));
+  return POYHIP_OK;
+}
+
 }  // extern "C"
 
 // ---------------------------------------------------------------------------

@@ -1,0 +1,367 @@
+"""``Sequence.readjust`` and ``DOS.readjust`` (src/sequence.ml:2097-2146,
+src/seqCS.ml:776-952) on the MI355X library: the pairwise centre search
+that re-optimises a node's median from its two children and its parent.
+
+``readjust`` / ``dos_readjust`` mirror the OCaml one node at a time through
+the drop-ins (``Align``, ``NewkkAlign``, ``poyhip_closest_2``).
+``readjust_level`` runs the same search for many nodes at once: four rounds
+of batched alignments (the three pairwise medians, the three
+median-against-third costs, the closest sequence of the chosen pair, the
+new median against children and parent) and batched column maps, with
+every sequence in one device arena; the host only plans the rounds and
+makes the comparisons the OCaml makes.  Nodes are dicts with "seq",
+"cost2", "cost2_max", "cost3" and "sum_cost", as DOS keeps them.
+
+One table serves as both ``c2_full`` and ``c2_original``: the two differ
+only for combination levels above 1, which the device tables do not take.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib
+from .batch import Batch
+from .cost_matrix import GAP
+from .sequence import Align, Context, NewkkAlign
+
+
+def is_empty(s) -> bool:
+    """Sequence.is_empty (sequence.ml:241): every symbol is the gap."""
+    s = np.asarray(s, np.uint8)
+    return len(s) == 0 or bool(np.all(s == GAP))
+
+
+def select_one(s):
+    """Sequence.select_one (sequence.ml:2069) for sets with combinations:
+    the lowest state of each symbol."""
+    s = np.asarray(s, np.int64)
+    if np.any(s == 0):
+        raise ValueError("Nothing?")
+    return (s & -s).astype(np.uint8)
+
+
+def _affine(ctx: Context) -> bool:
+    return ctx.cm is not None and ctx.cm.cost_model == 1
+
+
+def _algn(ctx: Context, s1, s2, use_ukk: bool):
+    """The `algn` of Sequence.readjust (sequence.ml:2099-2118): (cost,
+    median)."""
+    al = Align(ctx)
+    if _affine(ctx):
+        if use_ukk:
+            p1, p2, c = NewkkAlign(ctx).align_2(s1, s2)
+            return c, al.median_2(p1, p2)
+        m, _, _, c, _ = al.align_affine_3(s1, s2)
+        return c, m
+    if use_ukk:
+        raise ValueError("NewkkAlign under the linear model is not built (its fill is defective "
+                         "in the reference)")
+    p1, p2, c = al.align_2(s1, s2)
+    return c, al.median_2(p1, p2)
+
+
+def make_center(ctx: Context, a, b, c, use_ukk: bool = False):
+    """make_center (sequence.ml:2120-2136): the closest sequence to one of
+    the three from the median of the other two, the pair whose median is
+    cheapest to reach the third."""
+    cab, ab = _algn(ctx, a, b, use_ukk)
+    cbc, bc = _algn(ctx, b, c, use_ukk)
+    cac, ac = _algn(ctx, a, c, use_ukk)
+    cabc = _algn(ctx, ab, c, use_ukk)[0] + cab
+    cbca = _algn(ctx, bc, a, use_ukk)[0] + cbc
+    cacb = _algn(ctx, ac, b, use_ukk)[0] + cac
+    al = Align(ctx)
+    if cabc <= cbca:
+        if cabc <= cacb:
+            return al.closest(c, ab)
+        return al.closest(b, ac)
+    if cbca < cacb:
+        return al.closest(a, bc)
+    return al.closest(b, ac)
+
+
+def readjust(ctx: Context, a, b, m, parent, use_ukk: bool = False):
+    """Sequence.readjust (sequence.ml:2097-2146): (cost3, cost2, new median,
+    aliedmp, aliedmp, aliedmp) -- the aligned new median against the parent
+    three times, as the reference returns it."""
+    newseqm, _ = make_center(ctx, a, b, parent, use_ukk)
+    al = Align(ctx)
+    c1 = al.align_2(a, newseqm)[2]
+    c2 = al.align_2(b, newseqm)[2]
+    _, aliedmp, cp = al.align_2(parent, newseqm)
+    cost2 = c1 + c2
+    return cost2 + cp, cost2, newseqm, aliedmp, aliedmp, aliedmp
+
+
+def dos_distance(ctx: Context, a, b, use_ukk: bool, missing: int = 0) -> int:
+    """DOS.distance (seqCS.ml:701-775): band deltaw = max(|la - lb|, 8)."""
+    if is_empty(a) or is_empty(b):
+        return missing
+    deltaw = max(abs(len(a) - len(b)), 8)
+    if use_ukk:
+        return NewkkAlign(ctx).cost_2(a, b)
+    return Align(ctx).cost_2(a, b, deltaw)
+
+
+def _two_child(ctx: Context, s1, s2, use_ukk: bool):
+    """readjust_algn_two_child (seqCS.ml:783-813): (median, cost)."""
+    c, med = _algn(ctx, s1, s2, use_ukk)
+    return select_one(med), c
+
+
+def _node(seq, aligned, c2, c2max, c3, s):
+    return {"seq": seq, "aligned": aligned, "cost2": float(c2), "cost2_max": float(c2max),
+            "cost3": float(c3), "sum_cost": float(s)}
+
+
+def dos_readjust(ctx: Context, ch1, ch2, parent, mine, use_ukk: bool = False, _res=None):
+    """DOS.readjust (seqCS.ml:820-952), ApproxD mode: (changed, node, cost3,
+    cost2, sum_cost)."""
+    e1, e2, ep = is_empty(ch1["seq"]), is_empty(ch2["seq"]), is_empty(parent["seq"])
+    mc2, msum, mc3 = int(mine["cost2"]), int(mine["sum_cost"]), int(mine["cost3"])
+    sum12 = int(ch1["sum_cost"] + ch2["sum_cost"])
+
+    def differs(seq):
+        return not (len(seq) == len(mine["seq"]) and np.array_equal(seq, mine["seq"]))
+
+    if not e1 and not e2 and not ep:
+        c3, c2, m, al1, al2, mwg = (_res if _res is not None else
+                                    readjust(ctx, ch1["seq"], ch2["seq"], mine["seq"],
+                                             parent["seq"], use_ukk))
+        maxc2 = 0 if is_empty(al1) or is_empty(al2) else Align(ctx).max_cost_2(al1, al2)
+        s = c2 + sum12
+        changed = s != msum or c3 != mc3 or c2 != mc2 or differs(m)
+        return changed, _node(m, (al1, al2, mwg), c2, maxc2, c3, s), c3, c2, s
+    if (e1 and e2) or (e1 and ep) or (e2 and ep):
+        # the or-pattern binds the first alternative that matches
+        r = parent if (e1 and e2) else (ch2 if (e1 and ep) else ch1)
+        changed = sum12 != msum or 0 != mc3 or 0 != mc2 or differs(r["seq"])
+        out = dict(r)
+        out["sum_cost"] = float(sum12)
+        return changed, out, 0, 0, sum12
+    if ep:
+        med, c2 = _two_child(ctx, ch1["seq"], ch2["seq"], use_ukk)
+        c3 = c2
+    else:
+        ch = ch1 if e2 else ch2
+        med, _ = _two_child(ctx, ch["seq"], parent["seq"], use_ukk)
+        c2 = dos_distance(ctx, ch["seq"], med, use_ukk)
+        c3 = c2 + dos_distance(ctx, med, parent["seq"], use_ukk)
+    s = c2 + sum12
+    changed = s != msum or c3 != mc3 or c2 != mc2 or differs(med)
+    return changed, _node(med, (med, med, med), c2, 0, c3, s), c3, c2, s
+
+
+# ---- many nodes at once ---------------------------------------------------
+
+POST_JOB = np.dtype([("a", "<i8"), ("b", "<i8"), ("out", "<i8"), ("len", "<i4"),
+                     ("cap", "<i4")])
+
+
+class _Arena:
+    """One device byte blob that holds a level's inputs and every round's
+    outputs (bump allocation, 16-byte aligned), so each batch reads and
+    writes it through offsets."""
+
+    def __init__(self, nbytes: int, device):
+        self.t = torch.zeros(max(nbytes, 16), dtype=torch.uint8, device=device)
+        self.top = 0
+
+    def take(self, n: int) -> int:
+        at = (self.top + 15) & ~15
+        if at + n > self.t.numel():
+            raise MemoryError(f"readjust arena: {at + n} > {self.t.numel()} bytes")
+        self.top = at + n
+        return at
+
+    def put_all(self, hosts):
+        """Places host sequences in one transfer; returns [(at, len), ...]."""
+        base = self.take(0)
+        regs, o = [], 0
+        for h in hosts:
+            regs.append((base + o, len(h)))
+            o = (o + len(h) + 15) & ~15
+        self.take(o)
+        blob = np.zeros(max(o, 1), np.uint8)
+        for (a, n), h in zip(regs, hosts):
+            blob[a - base:a - base + n] = h
+        if o:
+            self.t[base:base + o].copy_(torch.from_numpy(blob[:o]))
+        return regs
+
+    def fetch(self, regions):
+        """Host copies of [(at, len), ...], gathered into one transfer."""
+        if not regions:
+            return []
+        flat = torch.cat([self.t[a:a + n] for a, n in regions]).cpu().numpy()
+        out, o = [], 0
+        for _, n in regions:
+            out.append(flat[o:o + n].copy())
+            o += n
+        return out
+
+
+class _Round:
+    """One batch of pairs over arena sequences: (offset, length) of each
+    side in, costs and the (offset, length) of each output region out."""
+
+    def __init__(self, ctx: Context, arena: _Arena, op: str, s1, s2):
+        self.ctx, self.arena = ctx, arena
+        dev = arena.t.device
+        l1 = np.array([n for _, n in s1], np.int32)
+        l2 = np.array([n for _, n in s2], np.int32)
+        self.b = Batch(ctx, op, l1, l2)
+        self.n = len(l1)
+        self.out_at = arena.take(self.b.out_bytes)
+        d1 = torch.from_numpy(np.array([o for o, _ in s1], np.int64)).to(dev)
+        d2 = torch.from_numpy(np.array([o for o, _ in s2], np.int64)).to(dev)
+        self.cost = torch.empty(self.n, dtype=torch.int32, device=dev)
+        self.outlen = torch.empty(4 * self.n, dtype=torch.int32, device=dev)
+        out = arena.t[self.out_at:self.out_at + max(self.b.out_bytes, 1)]
+        self.b.run(arena.t, d1, d2, self.cost, out, self.outlen, sync=True)
+        self.costs = self.cost.cpu().numpy().astype(np.int64)
+        self.lens = self.outlen.cpu().numpy().reshape(-1, 4)
+        L = self.b.L
+        self.off = np.array([L.poyhip_batch_out_offset(self.b.h, k) for k in range(self.n)],
+                            np.int64)
+        self.cap = np.array([L.poyhip_batch_out_cap(self.b.h, k) for k in range(self.n)],
+                            np.int64)
+        self.b.close()
+
+    def region(self, k: int, r: int):
+        n = int(self.lens[k, r])
+        base = self.out_at + int(self.off[k]) + r * int(self.cap[k])
+        return base + int(self.cap[k]) - n, n
+
+
+def _post(ctx: Context, arena: _Arena, op: int, pairs):
+    """Column maps of aligned pairs [((a_at, len), (b_at, len)), ...] into new
+    arena regions; returns [(at, len), ...]."""
+    n = len(pairs)
+    if n == 0:
+        return []
+    jobs = np.zeros(n, POST_JOB)
+    for k, ((a, la), (b, lb)) in enumerate(pairs):
+        if la != lb:
+            raise ValueError("aligned sequences of different lengths")
+        jobs[k] = (a, b, arena.take(la + 1), la, la + 1)
+    dev = arena.t.device
+    dj = torch.from_numpy(jobs.view(np.uint8).copy()).to(dev)
+    dl = torch.zeros(n, dtype=torch.int32, device=dev)
+    ds = torch.zeros(n, dtype=torch.int32, device=dev)
+    ctx.chk(ctx._lib.poyhip_pairs_post_batch(
+        ctx.h, op, n, C.c_void_p(dj.data_ptr()), C.c_void_p(arena.t.data_ptr()),
+        arena.t.numel(), C.c_void_p(dl.data_ptr()), C.c_void_p(ds.data_ptr()), 1))
+    ln, st = dl.cpu().numpy(), ds.cpu().numpy()
+    if np.any(st != _lib.POYHIP_OK):
+        raise ValueError("No bits on?" if op in (_lib.POST_CLOSEST, _lib.POST_CLOSEST_EQ)
+                         else "column map failed")
+    return [(int(jobs[k]["out"]) + int(jobs[k]["cap"]) - int(ln[k]), int(ln[k]))
+            for k in range(n)]
+
+
+def readjust_level(ctx: Context, triples, use_ukk: bool = False, device="cuda"):
+    """Sequence.readjust for many (a, b, parent) triples: a list of (cost3,
+    cost2, new median, aliedmp) as ``readjust`` gives them."""
+    n = len(triples)
+    if n == 0:
+        return []
+    affine = _affine(ctx)
+    if use_ukk and not affine:
+        raise ValueError("NewkkAlign under the linear model is not built (its fill is defective "
+                         "in the reference)")
+    align_op = "align_affine" if affine else "align_2d"
+    algn_op = "newkk_align" if use_ukk else align_op
+    seqs = [[np.asarray(x, np.uint8) for x in t] for t in triples]
+    tot = sum(len(a) + len(b) + len(c) for a, b, c in seqs)
+    # every round's outputs are bounded by a multiple of the node's three
+    # lengths (aligned pairs of at most twice the input, four regions each)
+    arena = _Arena(64 * tot + 1024 * n + 4096, device)
+    flat = arena.put_all([x for t in seqs for x in t])
+    at = [flat[3 * k:3 * k + 3] for k in range(n)]
+
+    def algn_round(pairs, medians=True):
+        """(cost, median region) of each pair, as `algn` computes them."""
+        r = _Round(ctx, arena, algn_op, [p for p, _ in pairs], [q for _, q in pairs])
+        if not medians:
+            return r.costs, None
+        if affine and not use_ukk:
+            return r.costs, [r.region(k, 2) for k in range(r.n)]
+        meds = _post(ctx, arena, _lib.POST_MEDIAN_2,
+                     [(r.region(k, 0), r.region(k, 1)) for k in range(r.n)])
+        return r.costs, meds
+
+    # round 1: the three pairwise medians of every node
+    p1 = []
+    for a, b, c in at:
+        p1 += [(a, b), (b, c), (a, c)]
+    c1, m1 = algn_round(p1)
+    # round 2: each median against the third sequence
+    p2 = []
+    for k, (a, b, c) in enumerate(at):
+        p2 += [(m1[3 * k], c), (m1[3 * k + 1], a), (m1[3 * k + 2], b)]
+    c2, _ = algn_round(p2, medians=False)
+    # make_center's comparisons, then Align.closest of the chosen pair
+    chosen = []
+    for k, (a, b, c) in enumerate(at):
+        cabc = c2[3 * k] + c1[3 * k]
+        cbca = c2[3 * k + 1] + c1[3 * k + 1]
+        cacb = c2[3 * k + 2] + c1[3 * k + 2]
+        if cabc <= cbca:
+            chosen.append((c, m1[3 * k]) if cabc <= cacb else (b, m1[3 * k + 2]))
+        elif cbca < cacb:
+            chosen.append((a, m1[3 * k + 1]))
+        else:
+            chosen.append((b, m1[3 * k + 2]))
+    newm = [None] * n
+    eq, al = [], []
+    for k, (h2, (s1, s2)) in enumerate(zip(arena.fetch([s2 for _, s2 in chosen]), chosen)):
+        h1 = seqs[k][at[k].index(s1)]  # the chosen third sequence
+        if is_empty(h2):
+            newm[k] = s2
+        elif len(h1) == len(h2) and np.array_equal(h1, h2):
+            eq.append(k)
+        else:
+            al.append(k)
+    for k, reg in zip(eq, _post(ctx, arena, _lib.POST_CLOSEST_EQ, [chosen[k] for k in eq])):
+        newm[k] = reg
+    if al:
+        r3 = _Round(ctx, arena, align_op, [chosen[k][0] for k in al], [chosen[k][1] for k in al])
+        regs = _post(ctx, arena, _lib.POST_CLOSEST,
+                     [(r3.region(q, 0), r3.region(q, 1)) for q in range(len(al))])
+        for k, reg in zip(al, regs):
+            newm[k] = reg
+    # round 4: the new median against both children and the parent
+    p4 = []
+    for k, (a, b, c) in enumerate(at):
+        p4 += [(a, newm[k]), (b, newm[k]), (c, newm[k])]
+    r4 = _Round(ctx, arena, align_op, [p for p, _ in p4], [q for _, q in p4])
+    hm = arena.fetch(newm)
+    hp = arena.fetch([r4.region(3 * k + 2, 1) for k in range(n)])
+    out = []
+    for k in range(n):
+        cost2 = int(r4.costs[3 * k] + r4.costs[3 * k + 1])
+        cost3 = cost2 + int(r4.costs[3 * k + 2])
+        out.append((cost3, cost2, hm[k], hp[k]))
+    return out
+
+
+def dos_readjust_level(ctx: Context, nodes, use_ukk: bool = False, device="cuda"):
+    """DOS.readjust for many (ch1, ch2, parent, mine) node tuples: nodes whose
+    three neighbours are all non-empty share one readjust_level; the others
+    take their branches one by one."""
+    full = [k for k, (c1, c2, p, _) in enumerate(nodes)
+            if not (is_empty(c1["seq"]) or is_empty(c2["seq"]) or is_empty(p["seq"]))]
+    res = readjust_level(ctx, [(nodes[k][0]["seq"], nodes[k][1]["seq"], nodes[k][2]["seq"])
+                               for k in full], use_ukk, device)
+    by_node = dict(zip(full, res))
+    out = []
+    for k, (c1, c2, p, m) in enumerate(nodes):
+        r = by_node.get(k)
+        pre = None if r is None else (r[0], r[1], r[2], r[3], r[3], r[3])
+        out.append(dos_readjust(ctx, c1, c2, p, m, use_ukk, _res=pre))
+    return out

@@ -262,6 +262,30 @@ class Align:
         self.ctx.chk(self.L.poyhip_worst_2(self.ctx.h, xp, yp, len(x), C.byref(out)))
         return out.value
 
+    def closest(self, s1, s2):
+        """Align.closest (sequence.ml:1180-1236) for the DNA tables (with
+        combinations, as every device table is): the states of s2 closest to s1 column by column along
+        their alignment, gaps removed; (sequence, cost_2 s1 sequence), cost
+        0 when s1 = s2 (no alignment) or s2 is all gaps (s2 returned)."""
+        s1 = np.asarray(s1, np.uint8)
+        s2 = np.asarray(s2, np.uint8)
+        if len(s2) == 0 or np.all(s2 == GAP):
+            return s2.copy(), 0
+        equal = len(s1) == len(s2) and np.array_equal(s1, s2)
+        a, b = (s1, s2) if equal else self.align_2(s1, s2)[:2]
+        x, xp = _u8(a)
+        y, yp = _u8(b)
+        cap = len(x) + 1
+        out = np.zeros(cap, np.uint8)
+        n = C.c_int()
+        rc = self.L.poyhip_closest_2(self.ctx.h, xp, yp, len(x), 1 if equal else 0,
+                                     out.ctypes.data_as(C.POINTER(C.c_uint8)), cap, C.byref(n))
+        if rc == -1:
+            raise ValueError("No bits on?")
+        self.ctx.chk(rc)
+        res = out[:n.value].copy()
+        return res, (0 if equal else self.cost_2(s1, res))
+
     def full_median_2(self, a, b):
         """Align.full_median_2 (sequence.ml:1136)."""
         if self.ctx.cm is not None and self.ctx.cm.cost_model == 1:

@@ -1,0 +1,210 @@
+"""GPU parity of the readjust path against the oracle, bit-exact: the column
+map of Align.closest (poyhip_closest_2 and its batched form beside
+median_2 / median_2_with_gaps), Align.closest, Sequence.readjust one node at
+a time and for many nodes in one level (src/sequence.ml:2097-2146), and
+DOS.readjust's branches (src/seqCS.ml:820-952), under linear and affine
+tables, with and without the diagonal-store aligner."""
+import ctypes as C
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from poy5_amd import _lib, cost_matrix, readjust
+from poy5_amd.sequence import Align, Context
+
+pytestmark = pytest.mark.gpu
+
+PLAIN = np.array([1, 2, 4, 8], np.uint8)
+IUPAC = np.array([3, 5, 9, 6, 10, 12, 7, 11, 13, 14, 15], np.uint8)
+GAPPY = np.array([17, 18, 20, 24, 31, 19, 28], np.uint8)
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = Context(0)
+    yield c
+    c.close()
+
+
+def rand_seq(rng, n, iupac=0.1):
+    s = rng.choice(PLAIN, n)
+    s = np.where(rng.random(n) < iupac, rng.choice(IUPAC, n), s)
+    return np.concatenate([[16], s]).astype(np.uint8)
+
+
+def mutate(rng, s, d):
+    out = [16]
+    for c in s[1:]:
+        u = rng.random()
+        if u < d / 3:
+            continue
+        if u < 2 * d / 3:
+            out.append(int(rng.choice(PLAIN)))
+        out.append(int(rng.choice(PLAIN)) if rng.random() < d else int(c))
+    return np.array(out, np.uint8)
+
+
+def aligned_pair(rng, n):
+    """Two equal-length columns of symbols, gaps and gap-carrying sets."""
+    pool = np.concatenate([PLAIN, IUPAC, GAPPY, [16, 16, 16]])
+    a = rng.choice(pool, n).astype(np.uint8)
+    b = rng.choice(pool, n).astype(np.uint8)
+    if n:
+        a[0] = b[0] = 16
+    return a, b
+
+
+def triples(seed, count, maxlen):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        root = rand_seq(rng, int(rng.integers(1, maxlen)))
+        out.append(tuple(mutate(rng, root, rng.uniform(0.02, 0.3)) for _ in range(3)))
+    x = rand_seq(rng, 20)
+    out += [(x, x, x), (x, x, rand_seq(rng, 20)), (rand_seq(rng, 1), x, x)]
+    return out
+
+
+def test_closest_dropin_matches_oracle(ctx):
+    say("closest drop-in")
+    ctx.set_cm(cost_matrix.of_transformations_and_gaps(1, 2))
+    ocm = oracle.cm_dna(1, 2)
+    rng = np.random.default_rng(4)
+    L = ctx._lib
+    for t in range(60):
+        a, b = aligned_pair(rng, int(rng.integers(1, 200)))
+        for eq in (0, 1):
+            x, y = (a, a) if eq else (a, b)
+            try:
+                want = oracle.closest_map(x, y, ocm, equal=bool(eq))
+            except ValueError:
+                want = None
+            out = np.zeros(len(x) + 1, np.uint8)
+            n = C.c_int()
+            rc = L.poyhip_closest_2(ctx.h, x.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                    y.ctypes.data_as(C.POINTER(C.c_uint8)), len(x), eq,
+                                    out.ctypes.data_as(C.POINTER(C.c_uint8)), len(out),
+                                    C.byref(n))
+            if want is None:
+                assert rc == -1, t  # POYHIP_EINVAL
+            else:
+                assert rc == 0 and np.array_equal(out[:n.value], want), (t, eq)
+
+
+@pytest.mark.parametrize("model", [(1, 2), (1, 1, 3)])
+def test_align_closest_matches_oracle(ctx, model):
+    say(f"Align.closest model={model}")
+    ctx.set_cm(cost_matrix.of_transformations_and_gaps(*model))
+    ocm = oracle.cm_dna(*model)
+    al = Align(ctx)
+    for t, (a, b, c) in enumerate(triples(8, 30, 200)):
+        r, cost = al.closest(a, b)
+        w, wc = oracle.closest(a, b, ocm)
+        assert np.array_equal(r, w) and cost == wc, t
+
+
+def test_batched_column_maps_match_oracle(ctx):
+    say("batched column maps")
+    ctx.set_cm(cost_matrix.of_transformations_and_gaps(1, 1, 3))
+    ocm = oracle.cm_dna(1, 1, 3)
+    rng = np.random.default_rng(6)
+    pairs = [aligned_pair(rng, int(n)) for n in rng.integers(1, 700, 300)]
+    pairs.append((np.zeros(0, np.uint8), np.zeros(0, np.uint8)))
+    pairs.append(aligned_pair(rng, 64))
+    pairs.append(aligned_pair(rng, 65))
+    blob, jobs, o = [], [], 0
+    for a, b in pairs:
+        n = len(a)
+        jobs.append((o, o + n, o + 2 * n, n, n + 1))
+        blob += [a, b, np.zeros(n + 1, np.uint8)]
+        o += 3 * n + 1
+    blob = np.concatenate(blob)
+    for op in (_lib.POST_MEDIAN_2, _lib.POST_MEDIAN_2_WITH_GAPS, _lib.POST_CLOSEST):
+        d_seq = torch.from_numpy(blob).cuda()
+        jj = np.zeros(len(jobs), readjust.POST_JOB)
+        for k, j in enumerate(jobs):
+            jj[k] = j
+        dj = torch.from_numpy(jj.view(np.uint8).copy()).cuda()
+        dl = torch.zeros(len(jobs), dtype=torch.int32, device="cuda")
+        ds = torch.zeros(len(jobs), dtype=torch.int32, device="cuda")
+        ctx.chk(ctx._lib.poyhip_pairs_post_batch(
+            ctx.h, op, len(jobs), C.c_void_p(dj.data_ptr()), C.c_void_p(d_seq.data_ptr()),
+            d_seq.numel(), C.c_void_p(dl.data_ptr()), C.c_void_p(ds.data_ptr()), 1))
+        hs, hl, hst = d_seq.cpu().numpy(), dl.cpu().numpy(), ds.cpu().numpy()
+        for k, ((a, b), j) in enumerate(zip(pairs, jobs)):
+            if op == _lib.POST_MEDIAN_2:
+                want = oracle.median_2_no_gaps(a, b, ocm)
+            elif op == _lib.POST_MEDIAN_2_WITH_GAPS:
+                want = oracle.median_2_with_gaps(a, b, ocm)
+            else:
+                try:
+                    want = oracle.closest_map(a, b, ocm)
+                except ValueError:
+                    want = None
+            if want is None:
+                assert hst[k] != 0, k
+                continue
+            assert hst[k] == 0, (op, k)
+            got = hs[j[2] + j[4] - hl[k]: j[2] + j[4]]
+            assert np.array_equal(got, want), (op, k, len(a))
+
+
+CASES = [((1, 2), False), ((1, 1, 3), False), ((1, 1, 3), True), ((2, 1, 4), False)]
+
+
+@pytest.mark.parametrize("model,ukk", CASES)
+def test_readjust_one_node_matches_oracle(ctx, model, ukk):
+    say(f"readjust per node model={model} ukk={ukk}")
+    ctx.set_cm(cost_matrix.of_transformations_and_gaps(*model))
+    ocm = oracle.cm_dna(*model)
+    for t, (a, b, p) in enumerate(triples(11 + len(model), 12, 150)):
+        got = readjust.readjust(ctx, a, b, a, p, ukk)
+        want = oracle.seq_readjust(a, b, a, p, ocm, ukk)
+        assert got[:2] == want[:2], t
+        assert np.array_equal(got[2], want[2]) and np.array_equal(got[3], want[3]), t
+
+
+@pytest.mark.parametrize("model,ukk", CASES)
+def test_readjust_level_matches_oracle(ctx, model, ukk):
+    say(f"readjust level model={model} ukk={ukk}")
+    ctx.set_cm(cost_matrix.of_transformations_and_gaps(*model))
+    ocm = oracle.cm_dna(*model)
+    tri = triples(21 + len(model), 60, 400)
+    got = readjust.readjust_level(ctx, tri, ukk)
+    for t, ((a, b, p), g) in enumerate(zip(tri, got)):
+        want = oracle.seq_readjust(a, b, a, p, ocm, ukk)
+        assert g[:2] == want[:2], t
+        assert np.array_equal(g[2], want[2]) and np.array_equal(g[3], want[3]), t
+
+
+def test_dos_readjust_level_branches_match_oracle(ctx):
+    say("DOS.readjust level with empty neighbours")
+    ctx.set_cm(cost_matrix.of_transformations_and_gaps(1, 2))
+    ocm = oracle.cm_dna(1, 2)
+    rng = np.random.default_rng(17)
+    e = np.array([16], np.uint8)
+
+    def node(s):
+        return {"seq": s, "cost2": float(rng.integers(0, 9)), "cost2_max": 0.0,
+                "cost3": float(rng.integers(0, 9)), "sum_cost": float(rng.integers(0, 20))}
+
+    nodes = []
+    for k, (a, b, p) in enumerate(triples(31, 40, 120)):
+        mask = k % 8  # bits: which of ch1, ch2, parent are emptied
+        a, b, p = (e if mask & 1 else a), (e if mask & 2 else b), (e if mask & 4 else p)
+        nodes.append((node(a), node(b), node(p), node(a)))
+    got = readjust.dos_readjust_level(ctx, nodes)
+    for t, ((c1, c2, p, m), g) in enumerate(zip(nodes, got)):
+        w = oracle.dos_readjust(c1, c2, p, m, ocm)
+        assert g[0] == w[0] and g[2:] == w[2:], t
+        assert np.array_equal(g[1]["seq"], w[1]["seq"]), t
+        for key in ("cost2", "cost2_max", "cost3", "sum_cost"):
+            assert g[1][key] == w[1][key], (t, key)
