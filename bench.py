@@ -53,7 +53,8 @@ def parse():
     ap.add_argument("--c4-pairs", type=int, default=2048)
     ap.add_argument("--c5-triples", type=int, default=16384)
     ap.add_argument("--nk-pairs", type=int, default=65536)
-    ap.add_argument("--only", choices=["c1", "c4", "c5", "nk", "un", "serve"], default=None,
+    ap.add_argument("--rj-nodes", type=int, default=4096)
+    ap.add_argument("--only", choices=["c1", "c4", "c5", "nk", "un", "rj", "serve"], default=None,
                     help="run one secondary configuration or the serving run alone (for "
                          "profiling) and print its record")
     ap.add_argument("--no-tune", action="store_true",
@@ -110,6 +111,16 @@ def cpu_job(job):
         L.orc_batch_align_2(len(l1), P(blob, C.c_uint8), P(o1, C.c_int64), P(l1, C.c_int),
                             P(o2, C.c_int64), P(l2, C.c_int), C.byref(cm), P(costs, C.c_int),
                             C.byref(cells))
+    elif kind == "readjust":
+        _, tcm, blob, o1, l1, o2, l2, o3, l3 = job
+        cm = oracle.cm_dna(*tcm)
+        costs = np.zeros(len(l1), np.int32)
+        t0 = time.perf_counter()
+        for k in range(len(l1)):
+            a = blob[o1[k]: o1[k] + l1[k]]
+            b = blob[o2[k]: o2[k] + l2[k]]
+            p = blob[o3[k]: o3[k] + l3[k]]
+            costs[k] = oracle.seq_readjust(a, b, a, p, cm)[0]
     else:
         _, tcm, blob, o1, l1, o2, l2, o3, l3 = job
         c3 = oracle.cm3_from_2d(oracle.cm_dna(*tcm))
@@ -161,7 +172,7 @@ def cpu_baseline(pool, procs, kind, tcm, blob, offs, lens, sample, gpu_costs, wh
     match = all(np.array_equal(c, gpu_costs[i]) for (_, c), i in zip(res, idx))
     cells = float(work.sum())
     return {"value": round(cells / wall / 1e9, 3), "unit": "GCUPS", "cores": len(jobs),
-            "kind": "port",
+            "kind": "port", "wall_s": round(wall, 3),
             "sample": f"{what} ({len(sample)} of {len(lens[0])}, {cells:.3e} full cells), "
                       f"oracle/oracle.c compiled -O2, {len(jobs)} processes, wall {wall:.2f} s",
             "costs_match_gpu": bool(match)}
@@ -289,6 +300,55 @@ def unions_workload(args, pool, procs):
                                "results_match_gpu": bool(same)}
     ctx.close()
     torch.cuda.empty_cache()
+    return rec
+
+
+def readjust_workload(args, pool, procs):
+    """DOS.readjust's pairwise centre search (Sequence.readjust,
+    src/sequence.ml:2097) for a level of nodes: config 5's triples as
+    (child, child, parent), its table (substitution 1, indel 1), ten
+    alignments and the closest map per node in four batched rounds.  The
+    level takes host sequences and returns host results (uploads, the
+    host's choices between rounds and downloads inside the timing)."""
+    import numpy as np
+    import torch
+    from poy5_amd import cost_matrix, readjust, workloads
+    from poy5_amd.sequence import Context
+    n = args.rj_nodes
+    print(f"[bench] readjust: {n} nodes", file=sys.stderr, flush=True)
+    ctx = Context(0)
+    ctx.set_cm(cost_matrix.of_transformations_and_gaps(1, 1))
+    blob, off, ln = workloads.gen_triples(n)
+    tri = [tuple(blob[off[3 * k + u]: off[3 * k + u] + ln[3 * k + u]] for u in range(3))
+           for k in range(n)]
+    readjust.readjust_level(ctx, tri[:64])  # first-call setup outside the timing
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = readjust.readjust_level(ctx, tri)
+    dt = time.perf_counter() - t0
+    # full cells of four of the ten alignments per node: the three input
+    # pairs and the new median against the three (the median-against-third
+    # and closest alignments are left out, so this bounds the work below)
+    l = [ln[u::3].astype(np.float64) for u in range(3)]
+    m = np.array([len(r[2]) for r in res], np.float64)
+    cells = float(np.sum(l[0] * l[1] + l[1] * l[2] + l[0] * l[2] + m * (l[0] + l[1] + l[2])))
+    rec = {"nodes": n, "ms": round(dt * 1e3, 3), "nodes_per_s": round(n / dt, 1),
+           "cells_lower_bound": cells, "gcups_lower_bound": round(cells / dt / 1e9, 2),
+           "rounds": 4, "alignments_per_node": 10}
+    gpu_cost3 = np.array([r[0] for r in res], np.int32)
+    ctx.close()
+    torch.cuda.empty_cache()
+    if pool is not None:
+        stride = 16
+        sample = np.arange(0, n, stride)
+        lens = [ln[u::3].astype(np.int32) for u in range(3)]
+        bl = cpu_baseline(pool, procs, "readjust", (1, 1), blob, [off[0::3], off[1::3], off[2::3]],
+                          lens, sample, gpu_cost3, f"every {stride}th node")
+        rec["cpu_baseline"] = {"value": round(len(sample) / bl["wall_s"], 2),
+                               "unit": "nodes/s", "cores": bl["cores"], "kind": "port",
+                               "sample": bl["sample"].replace("oracle/oracle.c",
+                                                              "oracle.seq_readjust over oracle/*.c"),
+                               "costs_match_gpu": bl["costs_match_gpu"]}
     return rec
 
 
@@ -508,6 +568,8 @@ def main():
     try:
         if args.only == "serve":
             print(json.dumps(serving(args.serve_requests)), flush=True)
+        elif args.only == "rj":
+            print(json.dumps(readjust_workload(args, pool, procs)), flush=True)
         elif args.only == "un":
             print(json.dumps(unions_workload(args, pool, procs)), flush=True)
         elif args.only == "nk":
@@ -677,6 +739,7 @@ def run_bench(args, pool, procs):
         result["secondary"] = secondary(args, pool, procs)
         result["secondary"]["newkk_config2"] = newkk_workload(args, pool, procs)
         result["secondary"]["union_merge_level"] = unions_workload(args, pool, procs)
+        result["secondary"]["readjust_level_config5"] = readjust_workload(args, pool, procs)
     if not args.no_serve:
         print("[bench] serving: 128-taxon SPR rejoin traffic", file=sys.stderr, flush=True)
         result["serving"] = serving(args.serve_requests)

@@ -297,6 +297,8 @@ int poyhip_batch_destroy(poyhip_batch *b);
 size_t poyhip_batch_out_bytes(const poyhip_batch *b);
 int64_t poyhip_batch_out_offset(const poyhip_batch *b, int k);
 int poyhip_batch_out_cap(const poyhip_batch *b, int k);
+/* Both of the above for every pair of the batch (n entries each). */
+int poyhip_batch_out_layout(const poyhip_batch *b, int64_t *off, int *cap);
 /* Runs the batch on device-resident inputs.  d_blob holds all sequences;
  * d_off1/d_off2 give the offsets of the caller's first and second sequence
  * of each pair.  Results: d_cost[n]; for align ops d_out (out_bytes) and

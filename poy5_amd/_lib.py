@@ -146,6 +146,7 @@ SIGNATURES = {
     "poyhip_batch_out_bytes": (C.c_size_t, [_vp]),
     "poyhip_batch_out_offset": (C.c_int64, [_vp, C.c_int]),
     "poyhip_batch_out_cap": (C.c_int, [_vp, C.c_int]),
+    "poyhip_batch_out_layout": (C.c_int, [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int)]),
     "poyhip_batch_run": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_int]),
     "poyhip_batch_stats": (C.c_int, [_vp, _i64p, _i64p, _ip, _ip]),
     "poyhip_batch_use_graph": (C.c_int, [_vp, C.c_int]),

@@ -1547,6 +1547,15 @@ int poyhip_batch_out_cap(const poyhip_batch* b, int k) {
   return 0;
 }
 
+int poyhip_batch_out_layout(const poyhip_batch* b, int64_t* off, int* cap) {
+  if (!b || !off || !cap) return POYHIP_EINVAL;
+  for (int k = 0; k < b->n; k++) {
+    off[k] = poyhip_batch_out_offset(b, k);
+    cap[k] = poyhip_batch_out_cap(b, k);
+  }
+  return POYHIP_OK;
+}
+
 int poyhip_set_tuning(poyhip_ctx* c, int knob, int value) {
   if (!c) return POYHIP_EINVAL;
   switch (knob) {

@@ -206,49 +206,48 @@ class _Arena:
 
 
 class _Round:
-    """One batch of pairs over arena sequences: (offset, length) of each
-    side in, costs and the (offset, length) of each output region out."""
+    """One batch of pairs over arena sequences: the two sides as (offset,
+    length) arrays in; costs and the output regions out."""
 
-    def __init__(self, ctx: Context, arena: _Arena, op: str, s1, s2):
-        self.ctx, self.arena = ctx, arena
+    def __init__(self, ctx: Context, arena: _Arena, op: str, at1, l1, at2, l2):
         dev = arena.t.device
-        l1 = np.array([n for _, n in s1], np.int32)
-        l2 = np.array([n for _, n in s2], np.int32)
-        self.b = Batch(ctx, op, l1, l2)
+        self.b = Batch(ctx, op, np.asarray(l1, np.int32), np.asarray(l2, np.int32))
         self.n = len(l1)
         self.out_at = arena.take(self.b.out_bytes)
-        d1 = torch.from_numpy(np.array([o for o, _ in s1], np.int64)).to(dev)
-        d2 = torch.from_numpy(np.array([o for o, _ in s2], np.int64)).to(dev)
-        self.cost = torch.empty(self.n, dtype=torch.int32, device=dev)
-        self.outlen = torch.empty(4 * self.n, dtype=torch.int32, device=dev)
+        d1 = torch.from_numpy(np.ascontiguousarray(at1, np.int64)).to(dev)
+        d2 = torch.from_numpy(np.ascontiguousarray(at2, np.int64)).to(dev)
+        cost = torch.empty(self.n, dtype=torch.int32, device=dev)
+        outlen = torch.empty(4 * self.n, dtype=torch.int32, device=dev)
         out = arena.t[self.out_at:self.out_at + max(self.b.out_bytes, 1)]
-        self.b.run(arena.t, d1, d2, self.cost, out, self.outlen, sync=True)
-        self.costs = self.cost.cpu().numpy().astype(np.int64)
-        self.lens = self.outlen.cpu().numpy().reshape(-1, 4)
-        L = self.b.L
-        self.off = np.array([L.poyhip_batch_out_offset(self.b.h, k) for k in range(self.n)],
-                            np.int64)
-        self.cap = np.array([L.poyhip_batch_out_cap(self.b.h, k) for k in range(self.n)],
-                            np.int64)
+        self.b.run(arena.t, d1, d2, cost, out, outlen, sync=True)
+        self.costs = cost.cpu().numpy().astype(np.int64)
+        self.lens = outlen.cpu().numpy().reshape(-1, 4).astype(np.int64)
+        self.off = np.zeros(self.n, np.int64)
+        cap = np.zeros(self.n, np.int32)
+        ctx.chk(self.b.L.poyhip_batch_out_layout(self.b.h,
+                                                 self.off.ctypes.data_as(C.POINTER(C.c_int64)),
+                                                 cap.ctypes.data_as(C.POINTER(C.c_int))))
+        self.cap = cap.astype(np.int64)
         self.b.close()
 
-    def region(self, k: int, r: int):
-        n = int(self.lens[k, r])
-        base = self.out_at + int(self.off[k]) + r * int(self.cap[k])
-        return base + int(self.cap[k]) - n, n
+    def region(self, r: int):
+        """(offsets, lengths) of output region r of every pair."""
+        n = self.lens[:, r]
+        return self.out_at + self.off + r * self.cap + self.cap - n, n
 
 
-def _post(ctx: Context, arena: _Arena, op: int, pairs):
-    """Column maps of aligned pairs [((a_at, len), (b_at, len)), ...] into new
-    arena regions; returns [(at, len), ...]."""
-    n = len(pairs)
+def _post(ctx: Context, arena: _Arena, op: int, at1, at2, ln):
+    """Column maps of aligned pairs (two offset arrays, one length array) into
+    new arena regions; returns (offsets, lengths)."""
+    n = len(ln)
     if n == 0:
-        return []
+        return np.zeros(0, np.int64), np.zeros(0, np.int64)
+    ln = np.asarray(ln, np.int64)
+    cap = ln + 1
+    size = (cap + 15) & ~15
+    out = arena.take(int(size.sum())) + np.concatenate([[0], np.cumsum(size)[:-1]])
     jobs = np.zeros(n, POST_JOB)
-    for k, ((a, la), (b, lb)) in enumerate(pairs):
-        if la != lb:
-            raise ValueError("aligned sequences of different lengths")
-        jobs[k] = (a, b, arena.take(la + 1), la, la + 1)
+    jobs["a"], jobs["b"], jobs["out"], jobs["len"], jobs["cap"] = at1, at2, out, ln, cap
     dev = arena.t.device
     dj = torch.from_numpy(jobs.view(np.uint8).copy()).to(dev)
     dl = torch.zeros(n, dtype=torch.int32, device=dev)
@@ -256,12 +255,11 @@ def _post(ctx: Context, arena: _Arena, op: int, pairs):
     ctx.chk(ctx._lib.poyhip_pairs_post_batch(
         ctx.h, op, n, C.c_void_p(dj.data_ptr()), C.c_void_p(arena.t.data_ptr()),
         arena.t.numel(), C.c_void_p(dl.data_ptr()), C.c_void_p(ds.data_ptr()), 1))
-    ln, st = dl.cpu().numpy(), ds.cpu().numpy()
+    got, st = dl.cpu().numpy().astype(np.int64), ds.cpu().numpy()
     if np.any(st != _lib.POYHIP_OK):
         raise ValueError("No bits on?" if op in (_lib.POST_CLOSEST, _lib.POST_CLOSEST_EQ)
                          else "column map failed")
-    return [(int(jobs[k]["out"]) + int(jobs[k]["cap"]) - int(ln[k]), int(ln[k]))
-            for k in range(n)]
+    return out + cap - got, got
 
 
 def readjust_level(ctx: Context, triples, use_ukk: bool = False, device="cuda"):
@@ -282,71 +280,72 @@ def readjust_level(ctx: Context, triples, use_ukk: bool = False, device="cuda"):
     # lengths (aligned pairs of at most twice the input, four regions each)
     arena = _Arena(64 * tot + 1024 * n + 4096, device)
     flat = arena.put_all([x for t in seqs for x in t])
-    at = [flat[3 * k:3 * k + 3] for k in range(n)]
+    at = np.array([o for o, _ in flat], np.int64).reshape(n, 3)   # a, b, parent
+    ln = np.array([q for _, q in flat], np.int64).reshape(n, 3)
 
-    def algn_round(pairs, medians=True):
-        """(cost, median region) of each pair, as `algn` computes them."""
-        r = _Round(ctx, arena, algn_op, [p for p, _ in pairs], [q for _, q in pairs])
+    def algn_round(at1, l1, at2, l2, medians=True):
+        """Costs and median regions of the pairs, as `algn` computes them."""
+        r = _Round(ctx, arena, algn_op, at1, l1, at2, l2)
         if not medians:
             return r.costs, None
         if affine and not use_ukk:
-            return r.costs, [r.region(k, 2) for k in range(r.n)]
-        meds = _post(ctx, arena, _lib.POST_MEDIAN_2,
-                     [(r.region(k, 0), r.region(k, 1)) for k in range(r.n)])
-        return r.costs, meds
+            return r.costs, r.region(2)
+        (x, n1), (y, _) = r.region(0), r.region(1)
+        return r.costs, _post(ctx, arena, _lib.POST_MEDIAN_2, x, y, n1)
 
-    # round 1: the three pairwise medians of every node
-    p1 = []
-    for a, b, c in at:
-        p1 += [(a, b), (b, c), (a, c)]
-    c1, m1 = algn_round(p1)
-    # round 2: each median against the third sequence
-    p2 = []
-    for k, (a, b, c) in enumerate(at):
-        p2 += [(m1[3 * k], c), (m1[3 * k + 1], a), (m1[3 * k + 2], b)]
-    c2, _ = algn_round(p2, medians=False)
-    # make_center's comparisons, then Align.closest of the chosen pair
-    chosen = []
-    for k, (a, b, c) in enumerate(at):
-        cabc = c2[3 * k] + c1[3 * k]
-        cbca = c2[3 * k + 1] + c1[3 * k + 1]
-        cacb = c2[3 * k + 2] + c1[3 * k + 2]
-        if cabc <= cbca:
-            chosen.append((c, m1[3 * k]) if cabc <= cacb else (b, m1[3 * k + 2]))
-        elif cbca < cacb:
-            chosen.append((a, m1[3 * k + 1]))
-        else:
-            chosen.append((b, m1[3 * k + 2]))
-    newm = [None] * n
+    def inter(*cols):
+        """Interleaves per-node columns into per-pair order (node-major)."""
+        return np.stack(cols, axis=1).reshape(-1)
+
+    # round 1: the pairwise medians (a, b), (b, c), (a, c) of every node
+    c1, (m_at, m_ln) = algn_round(inter(at[:, 0], at[:, 1], at[:, 0]),
+                                  inter(ln[:, 0], ln[:, 1], ln[:, 0]),
+                                  inter(at[:, 1], at[:, 2], at[:, 2]),
+                                  inter(ln[:, 1], ln[:, 2], ln[:, 2]))
+    m_at, m_ln, c1 = m_at.reshape(n, 3), m_ln.reshape(n, 3), c1.reshape(n, 3)
+    # round 2: each median against the third sequence: ab-c, bc-a, ac-b
+    third = np.array([2, 0, 1])
+    c2, _ = algn_round(m_at.reshape(-1), m_ln.reshape(-1), at[:, third].reshape(-1),
+                       ln[:, third].reshape(-1), medians=False)
+    tot3 = c2.reshape(n, 3) + c1          # cabc, cbca, cacb
+    cabc, cbca, cacb = tot3[:, 0], tot3[:, 1], tot3[:, 2]
+    # make_center's comparisons: which median (0 ab, 1 bc, 2 ac) against
+    # which sequence (2 c, 0 a, 1 b)
+    pick = np.where(cabc <= cbca, np.where(cabc <= cacb, 0, 2), np.where(cbca < cacb, 1, 2))
+    s1_at, s1_ln = at[np.arange(n), third[pick]], ln[np.arange(n), third[pick]]
+    s2_at, s2_ln = m_at[np.arange(n), pick], m_ln[np.arange(n), pick]
+    # Align.closest: an all-gap median is kept, equal sequences take the
+    # masked map, the others their alignment's map
+    h2 = arena.fetch(list(zip(s2_at, s2_ln)))
+    new_at, new_ln = s2_at.copy(), s2_ln.copy()
     eq, al = [], []
-    for k, (h2, (s1, s2)) in enumerate(zip(arena.fetch([s2 for _, s2 in chosen]), chosen)):
-        h1 = seqs[k][at[k].index(s1)]  # the chosen third sequence
-        if is_empty(h2):
-            newm[k] = s2
-        elif len(h1) == len(h2) and np.array_equal(h1, h2):
+    for k in range(n):
+        h1 = seqs[k][third[pick[k]]]
+        if is_empty(h2[k]):
+            continue
+        if len(h1) == len(h2[k]) and np.array_equal(h1, h2[k]):
             eq.append(k)
         else:
             al.append(k)
-    for k, reg in zip(eq, _post(ctx, arena, _lib.POST_CLOSEST_EQ, [chosen[k] for k in eq])):
-        newm[k] = reg
-    if al:
-        r3 = _Round(ctx, arena, align_op, [chosen[k][0] for k in al], [chosen[k][1] for k in al])
-        regs = _post(ctx, arena, _lib.POST_CLOSEST,
-                     [(r3.region(q, 0), r3.region(q, 1)) for q in range(len(al))])
-        for k, reg in zip(al, regs):
-            newm[k] = reg
-    # round 4: the new median against both children and the parent
-    p4 = []
-    for k, (a, b, c) in enumerate(at):
-        p4 += [(a, newm[k]), (b, newm[k]), (c, newm[k])]
-    r4 = _Round(ctx, arena, align_op, [p for p, _ in p4], [q for _, q in p4])
-    hm = arena.fetch(newm)
-    hp = arena.fetch([r4.region(3 * k + 2, 1) for k in range(n)])
+    eq, al = np.array(eq, np.int64), np.array(al, np.int64)
+    if len(eq):
+        new_at[eq], new_ln[eq] = _post(ctx, arena, _lib.POST_CLOSEST_EQ, s1_at[eq], s2_at[eq],
+                                       s1_ln[eq])
+    if len(al):
+        r3 = _Round(ctx, arena, align_op, s1_at[al], s1_ln[al], s2_at[al], s2_ln[al])
+        (x, n1), (y, _) = r3.region(0), r3.region(1)
+        new_at[al], new_ln[al] = _post(ctx, arena, _lib.POST_CLOSEST, x, y, n1)
+    # round 4: a, b and the parent against the new median
+    r4 = _Round(ctx, arena, align_op, at.reshape(-1), ln.reshape(-1), np.repeat(new_at, 3),
+                np.repeat(new_ln, 3))
+    c4 = r4.costs.reshape(n, 3)
+    p_at, p_ln = r4.region(1)
+    hm = arena.fetch(list(zip(new_at, new_ln)))
+    hp = arena.fetch(list(zip(p_at[2::3], p_ln[2::3])))
     out = []
     for k in range(n):
-        cost2 = int(r4.costs[3 * k] + r4.costs[3 * k + 1])
-        cost3 = cost2 + int(r4.costs[3 * k + 2])
-        out.append((cost3, cost2, hm[k], hp[k]))
+        cost2 = int(c4[k, 0] + c4[k, 1])
+        out.append((cost2 + int(c4[k, 2]), cost2, hm[k], hp[k]))
     return out
 
 
