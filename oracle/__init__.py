@@ -694,3 +694,171 @@ def dos_readjust(ch1, ch2, parent, mine, cm, use_ukk=False):
         c3 = c2 + dos_distance(med, parent["seq"], cm, use_ukk)
     changed = s != msum or c3 != mc3 or c2 != mc2 or differs(med)
     return changed, node(med, (med, med, med), c2, 0, c3, s), c3, c2, s
+
+
+# ---- Sequence.Clip (src/sequence.ml:3441-3575) and Unions.get_positions
+# (src/sequence.ml:2869-2904): OCaml bookkeeping around Align.align_2,
+# restated with the OCaml's own control flow ----
+
+def _get(s, pos):
+    if pos < 0 or pos >= len(s):
+        raise IndexError("Sequence.get out of range")
+    return int(s[pos])
+
+
+def clip_count_tip_gaps(seqa, seqb, gap=GAP):
+    """Clip.count_tip_gaps (src/sequence.ml:3443)."""
+    pos, acc = 0, 0
+    while True:
+        a, b = _get(seqa, pos), _get(seqb, pos)
+        if gap == a or gap == b:
+            if gap == a and gap == b:
+                pos += 1
+            else:
+                pos, acc = pos + 1, acc + 1
+        else:
+            return acc
+
+
+def clip_correct_distance(seqa, seqb, cost, cm, gap=GAP):
+    """Clip.correct_distance (src/sequence.ml:3455)."""
+    pos, acc = 0, 0
+    while gap == _get(seqa, pos) or gap == _get(seqb, pos):
+        acc += cm.cost[(_get(seqa, pos) << 5) + _get(seqb, pos)]
+        pos += 1
+    return cost if acc == 0 else cost - (acc + cm.gap_open)
+
+
+def clip_corrected_distance(a, b, cm, missing=0, use_ukk=False):
+    """Clip.corrected_distance (src/sequence.ml:3469)."""
+    if is_empty(a) or is_empty(b):
+        return missing
+    if use_ukk:
+        sa, sb, c = newkk_align_2(a, b, cm)
+    else:
+        sa, sb, c = align_2_any(a, b, cm)
+    return clip_correct_distance(sa, sb, c, cm)
+
+
+def _seq_init(f, n):
+    """Sequence.init (src/sequence.ml:159): f called from index n - 1 down to
+    0, each result prepended."""
+    out = []
+    for i in range(n - 1, -1, -1):
+        out.insert(0, f(i))
+    return np.array(out, np.uint8)
+
+
+def clip_n_fix(s1, s2):
+    """Clip.clip_n_fix (src/sequence.ml:3489-3539), fraction 1.00."""
+    def aux(s1, s2):
+        s1len, s2len = float(len(s1)), float(len(s2))
+        if s1len > s2len * 1.00:
+            maxlen = int(s2len * 1.00)
+            dif = int(s1len) - maxlen
+            cnt = [int(s1len)]
+
+            def f(x):
+                cnt[0] -= 1
+                return _get(s1, 0) if x == 0 else _get(s1, cnt[0])
+            s1p = _seq_init(f, maxlen)
+
+            def add_bottom(x):
+                return [int(v) for v in x[1:]]
+
+            def fixs1(x):
+                newx = add_bottom(x)
+                for i in range(dif, -1, -1):
+                    newx.insert(0, _get(s1, i))
+                return np.array(newx, np.uint8)
+
+            def fixs2(x):
+                newx = add_bottom(x)
+                g = _get(s1, 0)
+                for _ in range(dif, -1, -1):
+                    newx.insert(0, g)
+                return np.array(newx, np.uint8)
+            return s1p, np.asarray(s2, np.uint8), fixs1, fixs2, dif
+
+        def ident(x):
+            return np.asarray(x, np.uint8)
+        return np.asarray(s1, np.uint8), np.asarray(s2, np.uint8), ident, ident, 0
+
+    if len(s1) >= len(s2):
+        return aux(s1, s2)
+    a, b, c, d, dif = aux(s2, s1)
+    return b, a, d, c, dif
+
+
+def seq_safe_reverse(x):
+    """Sequence.safe_reverse (src/sequence.ml:186)."""
+    g = _get(x, 0)
+    y = []
+    for i in range(1, len(x)):
+        y.insert(0, int(x[i]))
+    return np.array([g] + y, np.uint8)
+
+
+def clip_align_2(s1, s2, cm, first_gap=True):
+    """Clip.Align.align_2 (src/sequence.ml:3541-3573) on (kind, sequence)."""
+    (k1, a), (k2, b) = s1, s2
+
+    def al2(x, y):
+        if first_gap:
+            return align_2_any(x, y, cm)
+        xx = np.concatenate([[GAP], x]).astype(np.uint8)
+        yy = np.concatenate([[GAP], y]).astype(np.uint8)
+        p, q, c = align_2_any(xx, yy, cm)
+        return p[1:], q[1:], c
+
+    def algn_and_fix(x, y):
+        x, y, f1, f2, dif = clip_n_fix(x, y)
+        x, y, cost = al2(x, y)
+        cost = clip_correct_distance(x, y, cost, cm)
+        tip = clip_count_tip_gaps(x, y)
+        return f1(x), f2(y), cost, dif + tip, x, x
+
+    if (k1, k2) == ("DO", "DO"):
+        x, y, cost = al2(a, b)
+        return ("DO", x), ("DO", y), cost, 0, ("DO", x), ("DO", y)
+    if (k1, k2) in (("Last", "DO"), ("DO", "Last"), ("Last", "Last")):
+        x, y, cost, dif, xo, yo = algn_and_fix(seq_safe_reverse(a), seq_safe_reverse(b))
+        r = seq_safe_reverse
+        return ("Last", r(x)), ("Last", r(y)), cost, dif, ("Last", r(xo)), ("Last", r(yo))
+    if (k1, k2) in (("First", "DO"), ("DO", "First"), ("First", "First")):
+        x, y, cost, dif, xo, yo = algn_and_fix(a, b)
+        return ("First", x), ("First", y), cost, dif, ("First", xo), ("First", yo)
+    raise AssertionError("assert false")
+
+
+def union_get_positions(u, positions):
+    """Unions.get_positions (src/sequence.ml:2869-2904)."""
+    ln = len(u.offset)
+    seq_len = u.length
+
+    def pos_finder(modifier, arr, x, guard=0):
+        if guard > 4 * ln + 8:
+            raise IndexError("no end")
+        if x == 0:
+            return int(arr[0])
+        if x == ln:
+            return pos_finder(-1, arr, x - 1, guard + 1)
+        if x < ln:
+            if x < 0:
+                raise IndexError("bigarray index")
+            if arr[x] != 0:
+                return int(arr[x])
+        return pos_finder(modifier, arr, x + modifier, guard + 1)
+
+    positions = list(positions)
+    poslen = len(positions) - 1
+
+    def get_position(arr, pos, xy):
+        x, y = xy
+        xmod = 1 if pos == poslen else -1
+        ymod = -1 if pos == 0 else 1
+        return (pos_finder(xmod, arr, ln - seq_len + x), pos_finder(ymod, arr, ln - seq_len + y))
+
+    a = [get_position(u.c1, k, p) for k, p in enumerate(positions)]
+    b = [get_position(u.c2, k, p) for k, p in enumerate(positions)]
+    return a, b
