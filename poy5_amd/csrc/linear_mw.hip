@@ -60,11 +60,20 @@ __device__ __forceinline__ void mw_stripe_geom(int b, int L1, int L2, int newk, 
   nsteps = (jhi - jlo + 1) + (ilast - i0);
 }
 
+// Row a of the packed table starts at 64 a + 8 (ctz(a) & 3): the rows of
+// A, C, G and T are rotated by 0, 8, 16 and 24 banks, so the 16 lookups of
+// two plain symbols fall in 16 different banks (with rows of 32 entries a
+// wavefront's lanes, which hold different row and column symbols, read the
+// same 4 banks).
+__device__ __forceinline__ int mw_ag_row(int a) {
+  return (a << 6) + (a ? ((__builtin_ctz(a) & 3) << 3) : 0);
+}
+
 // Shared memory of one workgroup.
 template <int NW>
 struct MwShared {
   int cost[1024];
-  int ag[1024];              // cost(a,b) | cost(gap,b) << 16
+  int ag[2048];              // cost(a,b) | cost(gap,b) << 16 at mw_ag_row(a) + b
   int prep[32];
   int tail[32];
   int2 src[NW][2][16];       // lane 0's row above, per block (double buffer)
@@ -206,7 +215,7 @@ __device__ __forceinline__ void lin_mw_pair(MwShared<NW>& S, const LinFillArgs& 
       }
       // per-stripe state (valid while b < nstripes)
       int i0 = 0, ilast = 0, jlo = 0, jhi = 0, nsteps = 0, nblk = 0;
-      int c = 0, tl = 0, col0 = 0, arow = 0, lo = 0, hi = 0;
+      int c = 0, tl = 0, col0 = 0, arow = 0, agrow = 0, lo = 0, hi = 0;
       bool act = false, hasL = false, hasR = false;
       int fast_lo = 0, fast_hi = -1;  // block starts (steps) of the fast path
       int dmv = 0, ddc = 0, lmv = 0, ldc = 0, omv = 0, odc = 0;
@@ -223,6 +232,7 @@ __device__ __forceinline__ void lin_mw_pair(MwShared<NW>& S, const LinFillArgs& 
         tl = s_tail[s1i];
         col0 = full ? c : tl;
         arow = s1i << 5;
+        agrow = mw_ag_row(s1i);
         lo = 1 << 30;
         hi = -1;
         hasL = hasR = false;
@@ -310,7 +320,7 @@ __device__ __forceinline__ void lin_mw_pair(MwShared<NW>& S, const LinFillArgs& 
                 src[2 * u + 1] = make_int2(v.z, v.w);
               }
 #pragma unroll
-              for (int u = 0; u < 8; u++) ag[u] = s_ag[arow + symw[8 * h + u]];
+              for (int u = 0; u < 8; u++) ag[u] = s_ag[agrow + symw[8 * h + u]];
 #pragma unroll
               for (int u = 0; u < 8; u++) {
                 const int2 sv = src[u];
@@ -484,7 +494,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) vo
   for (int k = tid; k < 1024; k += 64 * NW) {
     const int c = A.cm->cost[k];
     S.cost[k] = c;
-    S.ag[k] = (c & 0xFFFF) | (A.cm->cost[(kGap << 5) + (k & 31)] << 16);
+    S.ag[mw_ag_row(k >> 5) + (k & 31)] = (c & 0xFFFF) | (A.cm->cost[(kGap << 5) + (k & 31)] << 16);
   }
   if (tid < 32) {
     S.prep[tid] = A.cm->prepend[tid];
