@@ -346,9 +346,11 @@ int poyhip_union_merge_batch(poyhip_ctx *ctx, int n, const poyhip_union_job *d_j
  *   AFFINE_ROTATING   1: gap-free affine pairs use the rotating stripe
  *                     schedule within its limits (default); 0: the plain
  *                     stripe loop
- *   LINEAR_WORKGROUP  1: the linear fill runs a workgroup of wavefronts per
- *                     pair when every cost entry fits 16 bits (default);
- *                     0: one wavefront per pair
+ *   LINEAR_WORKGROUP  when every cost entry fits 16 bits, 2: the linear
+ *                     fill runs a workgroup of wavefronts per pair unless
+ *                     the batch has at least 16 pairs per CU, which run one
+ *                     wavefront each (default); 1: always a workgroup per
+ *                     pair; 0: always one wavefront per pair
  *   CUBE_WORKGROUP    1: the 3D fill runs a workgroup per triple when its
  *                     keys fit (default); 0: one wavefront per triple
  * Results do not depend on the knobs.  get returns the value or

@@ -50,7 +50,7 @@ def candidates(op: str) -> list[dict]:
         return [{"waves_per_cu": w, "affine_rotating": r}
                 for w, r in itertools.product((8, 12, 16), rot)]
     return [{"waves_per_cu": w, "linear_workgroup": g}
-            for w, g in itertools.product((8, 16), (1, 0))]
+            for w, g in itertools.product((8, 16), (2, 1, 0))]
 
 
 def _time_fill(ctx, op, d_blob, off1, off2, len1, len2, idx, repeats, device):

@@ -71,7 +71,7 @@ struct poyhip_ctx {
   bool has_cm3 = false;
   bool last3d = false;
   std::vector<uint8_t> last3[3];
-  // the linear fill runs one workgroup of wavefronts per pair when every
+  // the linear fill may run one workgroup of wavefronts per pair when every
   // 5-bit cost entry fits the packed 16-bit table (set by poyhip_set_cm);
   // POYHIP_LINEAR_FILL=wave forces the wavefront-per-pair kernel
   bool lin_mw = false;
@@ -83,7 +83,9 @@ struct poyhip_ctx {
   // workgroup linear fill (used when the costs also fit, lin_fits)
   int waves_per_cu = 16;
   bool aff_rot = true;
-  bool lin_wg = true;
+  // 2: a workgroup per pair unless the batch alone gives every SIMD four
+  // wavefronts of its own pairs; 1: a workgroup per pair; 0: a wavefront
+  int lin_wg = 2;
   bool lin_fits = false;
   // largest entry of the 3D cost table, -1 when an entry is negative (the
   // workgroup fill carries 8*value + code keys and needs values below 2^27)
@@ -279,7 +281,12 @@ int64_t lin_ws_slot_ints(int Lb) {
 hipError_t launch_linear_fill(poyhip_ctx* c, const LinFillArgs& A, bool dm, int slots,
                               hipStream_t st) {
   const int n = A.pt.n;
-  if (c->lin_mw) {
+  // with four of its own pairs per SIMD a batch keeps the device busy with
+  // one wavefront per pair, which saves the workgroup's block barriers and
+  // stripe schedule (a level of ~600-symbol pairs runs ~1.5x faster); the
+  // workgroup pays off when fewer, longer pairs must fill the device
+  const bool mw = c->lin_mw && (c->lin_wg == 1 || n < 16 * c->num_cus);
+  if (mw) {
     const int grid = std::max(1, std::min({n, slots, c->num_cus * kLinMwPerCu}));
     const dim3 blk(64 * kLinMwWaves);
     if (A.phase == 2)
@@ -483,7 +490,7 @@ int poyhip_create(int device, poyhip_ctx** out) {
   const char* e = getenv("POYHIP_CUBE_FILL");
   c->cube_mw = !(e && strcmp(e, "wave") == 0);
   e = getenv("POYHIP_LINEAR_FILL");
-  c->lin_wg = !(e && strcmp(e, "wave") == 0);
+  if (e && strcmp(e, "wave") == 0) c->lin_wg = 0;
   e = getenv("POYHIP_NEWKK_COUNTS");
   c->nk_wide_always = e && strcmp(e, "both") == 0;
   e = getenv("POYHIP_UNION_LANES");
@@ -557,7 +564,7 @@ int poyhip_set_cm(poyhip_ctx* c, const poyhip_cm* cm) {
   bool fits = true;
   for (int k = 0; k < 1024; k++) fits &= cm->cost[k] >= 0 && cm->cost[k] < 32768;
   c->lin_fits = fits;
-  c->lin_mw = fits && c->lin_wg;
+  c->lin_mw = fits && c->lin_wg != 0;
   int mx = 0;
   bool neg = false;
   for (int k = 0; k < 1024; k++) {
@@ -1567,8 +1574,9 @@ int poyhip_set_tuning(poyhip_ctx* c, int knob, int value) {
       c->aff_rot = value != 0;
       return POYHIP_OK;
     case POYHIP_TUNE_LINEAR_WORKGROUP:
-      c->lin_wg = value != 0;
-      c->lin_mw = c->lin_fits && c->lin_wg;
+      if (value < 0 || value > 2) return set_err(c, POYHIP_EINVAL, "linear workgroup %d", value);
+      c->lin_wg = value;
+      c->lin_mw = c->lin_fits && c->lin_wg != 0;
       return POYHIP_OK;
     case POYHIP_TUNE_CUBE_WORKGROUP:
       c->cube_mw = value != 0;
@@ -1583,7 +1591,7 @@ int poyhip_get_tuning(poyhip_ctx* c, int knob) {
   switch (knob) {
     case POYHIP_TUNE_WAVES_PER_CU: return c->waves_per_cu;
     case POYHIP_TUNE_AFFINE_ROTATING: return c->aff_rot ? 1 : 0;
-    case POYHIP_TUNE_LINEAR_WORKGROUP: return c->lin_wg ? 1 : 0;
+    case POYHIP_TUNE_LINEAR_WORKGROUP: return c->lin_wg;
     case POYHIP_TUNE_CUBE_WORKGROUP: return c->cube_mw ? 1 : 0;
     default: return POYHIP_EINVAL;
   }
