@@ -68,7 +68,8 @@ def parse():
 
 
 def host_cores():
-    """Cores this process may use, capped at the 16 a one-GPU box allots."""
+    """Cores this process may use, capped at 16 (the host cores that go with
+    one GPU on the measurement nodes)."""
     try:
         n = len(os.sched_getaffinity(0))
     except AttributeError:

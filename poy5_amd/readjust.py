@@ -297,13 +297,13 @@ def readjust_level(ctx: Context, triples, use_ukk: bool = False, device="cuda"):
         """Interleaves per-node columns into per-pair order (node-major)."""
         return np.stack(cols, axis=1).reshape(-1)
 
-    # round 1: the pairwise medians (a, b), (b, c), (a, c) of every node
+    # first batch: the pairwise medians (a, b), (b, c), (a, c) of every node
     c1, (m_at, m_ln) = algn_round(inter(at[:, 0], at[:, 1], at[:, 0]),
                                   inter(ln[:, 0], ln[:, 1], ln[:, 0]),
                                   inter(at[:, 1], at[:, 2], at[:, 2]),
                                   inter(ln[:, 1], ln[:, 2], ln[:, 2]))
     m_at, m_ln, c1 = m_at.reshape(n, 3), m_ln.reshape(n, 3), c1.reshape(n, 3)
-    # round 2: each median against the third sequence: ab-c, bc-a, ac-b
+    # second batch: each median against the third sequence: ab-c, bc-a, ac-b
     third = np.array([2, 0, 1])
     c2, _ = algn_round(m_at.reshape(-1), m_ln.reshape(-1), at[:, third].reshape(-1),
                        ln[:, third].reshape(-1), medians=False)
@@ -335,7 +335,7 @@ def readjust_level(ctx: Context, triples, use_ukk: bool = False, device="cuda"):
         r3 = _Round(ctx, arena, align_op, s1_at[al], s1_ln[al], s2_at[al], s2_ln[al])
         (x, n1), (y, _) = r3.region(0), r3.region(1)
         new_at[al], new_ln[al] = _post(ctx, arena, _lib.POST_CLOSEST, x, y, n1)
-    # round 4: a, b and the parent against the new median
+    # last batch: a, b and the parent against the new median
     r4 = _Round(ctx, arena, align_op, at.reshape(-1), ln.reshape(-1), np.repeat(new_at, 3),
                 np.repeat(new_ln, 3))
     c4 = r4.costs.reshape(n, 3)

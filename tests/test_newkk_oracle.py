@@ -163,8 +163,9 @@ def test_newkk_band_never_beats_full_matrix():
 
 def test_newkk_hand_cases():
     cm = oracle.cm_dna(1, 1, 3)
-    # -A vs -AA: round 1 (k = 0) and round 2 (k = 1) both price 4 (one
-    # aligned A, one gap of go 3 + 1); at (1, 2) insert and align tie and
+    # -A vs -AA: the first doubling step (k = 0) and the second (k = 1)
+    # both price 4 (one aligned A, one gap of go 3 + 1); at (1, 2) insert
+    # and align tie and
     # both swap orders take the insertion first, so the gap is the last
     # column.
     for sw in (0, 1):
