@@ -589,6 +589,20 @@ def closest(s1, s2, cm):
     return r, cost_2_any(s1, r, cm)
 
 
+def newkk_closest(s1, s2, cm):
+    """NewkkAlign.closest (src/sequence.ml:1997-2060): closest with
+    NewkkAlign.align_2 and NewkkAlign.cost_2."""
+    s1 = np.asarray(s1, np.uint8)
+    s2 = np.asarray(s2, np.uint8)
+    if is_empty(s2):
+        return s2.copy(), 0
+    if len(s1) == len(s2) and np.array_equal(s1, s2):
+        return closest_map(s1, s2, cm, equal=True), 0
+    p1, p2, _ = newkk_align_2(s1, s2, cm)
+    r = closest_map(p1, p2, cm)
+    return r, newkk_cost_2(s1, r, cm)
+
+
 def make_center(a, b, c, cm, use_ukk):
     """make_center of Sequence.readjust (src/sequence.ml:2120-2136)."""
     cab, ab = readjust_algn(a, b, cm, use_ukk)

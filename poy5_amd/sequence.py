@@ -57,6 +57,22 @@ class Context:
         return _lib.check(rc, self.h)
 
 
+def _closest_map(ctx: Context, a, b, equal: bool):
+    """The column map of Align.closest / NewkkAlign.closest on the device
+    (poyhip_closest_2)."""
+    x, xp = _u8(a)
+    y, yp = _u8(b)
+    cap = len(x) + 1
+    out = np.zeros(cap, np.uint8)
+    n = C.c_int()
+    rc = ctx._lib.poyhip_closest_2(ctx.h, xp, yp, len(x), 1 if equal else 0,
+                                   out.ctypes.data_as(C.POINTER(C.c_uint8)), cap, C.byref(n))
+    if rc == -1:
+        raise ValueError("No bits on?")
+    ctx.chk(rc)
+    return out[:n.value].copy()
+
+
 class Align:
     """``Sequence.Align`` over a device context."""
 
@@ -273,17 +289,7 @@ class Align:
             return s2.copy(), 0
         equal = len(s1) == len(s2) and np.array_equal(s1, s2)
         a, b = (s1, s2) if equal else self.align_2(s1, s2)[:2]
-        x, xp = _u8(a)
-        y, yp = _u8(b)
-        cap = len(x) + 1
-        out = np.zeros(cap, np.uint8)
-        n = C.c_int()
-        rc = self.L.poyhip_closest_2(self.ctx.h, xp, yp, len(x), 1 if equal else 0,
-                                     out.ctypes.data_as(C.POINTER(C.c_uint8)), cap, C.byref(n))
-        if rc == -1:
-            raise ValueError("No bits on?")
-        self.ctx.chk(rc)
-        res = out[:n.value].copy()
+        res = _closest_map(self.ctx, a, b, equal)
         return res, (0 if equal else self.cost_2(s1, res))
 
     def full_median_2(self, a, b):
@@ -409,3 +415,15 @@ class NewkkAlign:
         if len(b) == 0:
             b = np.array([GAP], np.uint8)
         return self.newkk_cost2_affine(a, b, 0)
+
+    def closest(self, s1, s2):
+        """NewkkAlign.closest (sequence.ml:1997-2060): Align.closest with
+        this module's align_2 and cost_2."""
+        s1 = np.asarray(s1, np.uint8)
+        s2 = np.asarray(s2, np.uint8)
+        if len(s2) == 0 or np.all(s2 == GAP):
+            return s2.copy(), 0
+        equal = len(s1) == len(s2) and np.array_equal(s1, s2)
+        a, b = (s1, s2) if equal else self.align_2(s1, s2)[:2]
+        res = _closest_map(self.ctx, a, b, equal)
+        return res, (0 if equal else self.cost_2(s1, res))

@@ -111,6 +111,19 @@ def test_align_closest_matches_oracle(ctx, model):
         assert np.array_equal(r, w) and cost == wc, t
 
 
+def test_newkk_closest_matches_oracle(ctx):
+    say("NewkkAlign.closest")
+    from poy5_amd.sequence import NewkkAlign
+    ctx.set_cm(cost_matrix.of_transformations_and_gaps(1, 1, 3))
+    ocm = oracle.cm_dna(1, 1, 3)
+    nk = NewkkAlign(ctx)
+    for t, (a, b, c) in enumerate(triples(12, 30, 200)):
+        for x, y in ((a, b), (c, c), (a, np.array([16, 16], np.uint8))):
+            r, cost = nk.closest(x, y)
+            w, wc = oracle.newkk_closest(x, y, ocm)
+            assert np.array_equal(r, w) and cost == wc, t
+
+
 def test_batched_column_maps_match_oracle(ctx):
     say("batched column maps")
     ctx.set_cm(cost_matrix.of_transformations_and_gaps(1, 1, 3))
