@@ -179,30 +179,42 @@ class _Arena:
         return at
 
     def put_all(self, hosts):
-        """Places host sequences in one transfer; returns [(at, len), ...]."""
+        """Places host sequences in one transfer, each at a 16-byte boundary;
+        returns [(at, len), ...]."""
         base = self.take(0)
-        regs, o = [], 0
-        for h in hosts:
-            regs.append((base + o, len(h)))
-            o = (o + len(h) + 15) & ~15
+        ln = np.array([len(h) for h in hosts], np.int64)
+        size = (ln + 15) & ~15
+        rel = np.zeros(len(hosts), np.int64)
+        if len(hosts) > 1:
+            np.cumsum(size[:-1], out=rel[1:])
+        o = int(size.sum())
         self.take(o)
-        blob = np.zeros(max(o, 1), np.uint8)
-        for (a, n), h in zip(regs, hosts):
-            blob[a - base:a - base + n] = h
         if o:
-            self.t[base:base + o].copy_(torch.from_numpy(blob[:o]))
-        return regs
+            parts = []
+            for h, n, sz in zip(hosts, ln, size):
+                parts.append(np.asarray(h, np.uint8))
+                if sz > n:
+                    parts.append(np.zeros(int(sz - n), np.uint8))
+            self.t[base:base + o].copy_(torch.from_numpy(np.concatenate(parts)))
+        return [(base + int(r), int(n)) for r, n in zip(rel, ln)]
 
-    def fetch(self, regions):
-        """Host copies of [(at, len), ...], gathered into one transfer."""
-        if not regions:
-            return []
-        flat = torch.cat([self.t[a:a + n] for a, n in regions]).cpu().numpy()
-        out, o = [], 0
-        for _, n in regions:
-            out.append(flat[o:o + n].copy())
-            o += n
-        return out
+    def fetch(self, at, ln):
+        """Host copies of the regions (offsets at, lengths ln): the bytes are
+        gathered on the device into one buffer and copied over at once;
+        returns (flat bytes, start of each region in them)."""
+        at = np.asarray(at, np.int64)
+        ln = np.asarray(ln, np.int64)
+        starts = np.zeros(len(ln), np.int64)
+        if len(ln) > 1:
+            np.cumsum(ln[:-1], out=starts[1:])
+        total = int(ln.sum())
+        if total == 0:
+            return np.zeros(0, np.uint8), starts
+        dev = self.t.device
+        base = torch.from_numpy(at - starts).to(dev)
+        idx = torch.repeat_interleave(base, torch.from_numpy(ln).to(dev), output_size=total)
+        idx += torch.arange(total, dtype=torch.int64, device=dev)
+        return self.t[idx].cpu().numpy(), starts
 
 
 class _Round:
@@ -316,14 +328,15 @@ def readjust_level(ctx: Context, triples, use_ukk: bool = False, device="cuda"):
     s2_at, s2_ln = m_at[np.arange(n), pick], m_ln[np.arange(n), pick]
     # Align.closest: an all-gap median is kept, equal sequences take the
     # masked map, the others their alignment's map
-    h2 = arena.fetch(list(zip(s2_at, s2_ln)))
+    flat, st = arena.fetch(s2_at, s2_ln)
+    # symbols other than the gap per median
+    run = np.concatenate([[0], np.cumsum(flat != GAP)])
+    empty = run[st + s2_ln] - run[st] == 0
     new_at, new_ln = s2_at.copy(), s2_ln.copy()
     eq, al = [], []
-    for k in range(n):
-        h1 = seqs[k][third[pick[k]]]
-        if is_empty(h2[k]):
-            continue
-        if len(h1) == len(h2[k]) and np.array_equal(h1, h2[k]):
+    for k in np.nonzero(~empty)[0]:
+        h1 = seqs[k][third[pick[k]]]  # the chosen third sequence
+        if len(h1) == s2_ln[k] and np.array_equal(h1, flat[st[k]:st[k] + s2_ln[k]]):
             eq.append(k)
         else:
             al.append(k)
@@ -340,12 +353,14 @@ def readjust_level(ctx: Context, triples, use_ukk: bool = False, device="cuda"):
                 np.repeat(new_ln, 3))
     c4 = r4.costs.reshape(n, 3)
     p_at, p_ln = r4.region(1)
-    hm = arena.fetch(list(zip(new_at, new_ln)))
-    hp = arena.fetch(list(zip(p_at[2::3], p_ln[2::3])))
+    fm, sm = arena.fetch(new_at, new_ln)
+    pl = p_ln[2::3]
+    fp, sp = arena.fetch(p_at[2::3], pl)
     out = []
     for k in range(n):
         cost2 = int(c4[k, 0] + c4[k, 1])
-        out.append((cost2 + int(c4[k, 2]), cost2, hm[k], hp[k]))
+        out.append((cost2 + int(c4[k, 2]), cost2, fm[sm[k]:sm[k] + new_ln[k]].copy(),
+                    fp[sp[k]:sp[k] + pl[k]].copy()))
     return out
 
 
