@@ -198,6 +198,30 @@ def test_readjust_level_matches_oracle(ctx, model, ukk):
         assert np.array_equal(g[2], want[2]) and np.array_equal(g[3], want[3]), t
 
 
+@pytest.mark.parametrize("model,ukk", [((1, 1, 3), False), ((1, 1, 3), True), ((1, 2), False)])
+def test_readjust_level_with_gap_bit_symbols(ctx, model, ukk):
+    # medians of affine trees carry gap-bit sets (A|gap and the like): the
+    # batches route such pairs through the general kernels
+    say(f"readjust level gap-bit symbols model={model} ukk={ukk}")
+    ctx.set_cm(cost_matrix.of_transformations_and_gaps(*model))
+    ocm = oracle.cm_dna(*model)
+    rng = np.random.default_rng(41)
+    tri = []
+    for t in range(40):
+        root = rand_seq(rng, int(rng.integers(1, 300)))
+        seqs = [mutate(rng, root, rng.uniform(0.02, 0.2)) for _ in range(3)]
+        for s_ in seqs:
+            m = rng.random(len(s_)) < 0.1
+            m[0] = False
+            s_[m] = rng.choice(GAPPY, int(m.sum()))
+        tri.append(tuple(seqs))
+    got = readjust.readjust_level(ctx, tri, ukk)
+    for t, ((a, b, p), g) in enumerate(zip(tri, got)):
+        want = oracle.seq_readjust(a, b, a, p, ocm, ukk)
+        assert g[:2] == want[:2], t
+        assert np.array_equal(g[2], want[2]) and np.array_equal(g[3], want[3]), t
+
+
 def test_dos_readjust_level_branches_match_oracle(ctx):
     say("DOS.readjust level with empty neighbours")
     ctx.set_cm(cost_matrix.of_transformations_and_gaps(1, 2))
