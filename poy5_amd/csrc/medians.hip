@@ -205,7 +205,7 @@ __global__ __launch_bounds__(256) void k_pairs_post(const CmDev* cm, int op,
   if (k >= n) return;
   const poyhip_post_job J = jobs[k];
   const bool gaps_out = op != kMedianWithGaps;  // a leading gap is prepended
-  if (!(J.len >= 0 && J.cap >= J.len + 1 && post_in(J.a, J.len, seq_bytes) &&
+  if (!(J.len >= 0 && J.cap > J.len && post_in(J.a, J.len, seq_bytes) &&
         post_in(J.b, J.len, seq_bytes) && post_in(J.out, J.cap, seq_bytes))) {
     if (lane == 0) {
       out_len[k] = 0;
