@@ -435,16 +435,32 @@ __global__ __launch_bounds__(64) void k_linear_backtrace(LinBtArgs A) {
 
 // Align.count_gaps (seq_CAML_count, src/seq.c:644) for both sequences and
 // Align.cost_2's deltawh (src/sequence.ml:868-936).  has_deltaw = 0 is the
-// ?deltaw=None call made by align_2; 1 is DOS.distance's ~deltaw.
-__global__ void k_linear_deltawh(PairTable pt, int* deltawh, int has_deltaw, const int* deltaw) {
-  const int pid = blockIdx.x * blockDim.x + threadIdx.x;
+// ?deltaw=None call made by align_2; 1 is DOS.distance's ~deltaw.  One
+// wavefront per pair (kDwPairsPerBlock a block): lanes count consecutive
+// bytes, a wavefront sum gives the counts.
+constexpr int kDwPairsPerBlock = 4;
+
+__global__ __launch_bounds__(64 * kDwPairsPerBlock) void k_linear_deltawh(PairTable pt,
+                                                                       int* deltawh,
+                                                                       int has_deltaw,
+                                                                       const int* deltaw) {
+  const int pid = blockIdx.x * kDwPairsPerBlock + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
   if (pid >= pt.npairs) return;
   const uint8_t* sa = pt.blob + rows_off(pt, pid);
   const uint8_t* sb = pt.blob + cols_off(pt, pid);
   const int L1 = pt.len_a[pid], L2 = pt.len_b[pid];
+  // a symbol counts when it carries the gap bit
   int g1 = 0, g2 = 0;
-  for (int k = 0; k < L1; k++) g1 += (sa[k] >= kGap && (sa[k] & kGap)) ? 1 : 0;
-  for (int k = 0; k < L2; k++) g2 += (sb[k] >= kGap && (sb[k] & kGap)) ? 1 : 0;
+#pragma unroll 4
+  for (int k = lane; k < L1; k += 64) g1 += (sa[k] >> 4) & 1;
+#pragma unroll 4
+  for (int k = lane; k < L2; k += 64) g2 += (sb[k] >> 4) & 1;
+  for (int o = 32; o > 0; o >>= 1) {
+    g1 += __shfl_xor(g1, o);
+    g2 += __shfl_xor(g2, o);
+  }
+  if (lane != 0) return;
   const int dif = L1 - L2;
   const int lower = L1 / 10;
   // has_deltaw: 0 = ?deltaw=None for every pair, 1 = ~deltaw for every

@@ -1266,7 +1266,8 @@ int batch_enqueue(poyhip_batch* b, const uint8_t* blob, const int64_t* off1, con
   }
   if (op_is_linear(b->op)) {
     PairTable all = batch_table(b, blob, off1, off2, -1);
-    hipLaunchKernelGGL(k_linear_deltawh, dim3((b->n + 255) / 256), dim3(256), 0, st, all, b->d_dw,
+    hipLaunchKernelGGL(k_linear_deltawh, dim3((b->n + kDwPairsPerBlock - 1) / kDwPairsPerBlock),
+                       dim3(64 * kDwPairsPerBlock), 0, st, all, b->d_dw,
                        b->has_deltaw, b->d_dwin);
   }
   for (int k = 0; k < nch; k++) {
@@ -1632,7 +1633,8 @@ int batch_run_two_phase(poyhip_batch* b, const uint8_t* d_blob, const int64_t* d
   PairTable all = batch_table(b, d_blob, d_off1, d_off2, -1);
   const bool nk = b->op == POYHIP_OP_NEWKK_ALIGN;
   if (!nk)
-    hipLaunchKernelGGL(k_linear_deltawh, dim3((n + 255) / 256), dim3(256), 0, st, all, b->d_dw,
+    hipLaunchKernelGGL(k_linear_deltawh, dim3((n + kDwPairsPerBlock - 1) / kDwPairsPerBlock),
+                       dim3(64 * kDwPairsPerBlock), 0, st, all, b->d_dw,
                        b->has_deltaw, b->d_dwin);
   LinFillArgs A;
   A.pt = all;

@@ -298,7 +298,8 @@ int srv_launch(poyhip_server* s, SrvSlot& sl) {
   if (op == POYHIP_OP_ALIGN_AFFINE)
     hipLaunchKernelGGL(k_affine_classify, dim3(n), dim3(64), 0, st, pt, sl.d_cls, nullptr);
   if (!op_is_affine(op))
-    hipLaunchKernelGGL(k_linear_deltawh, dim3((n + 255) / 256), dim3(256), 0, st, pt, dt.dw, 2,
+    hipLaunchKernelGGL(k_linear_deltawh, dim3((n + kDwPairsPerBlock - 1) / kDwPairsPerBlock),
+                       dim3(64 * kDwPairsPerBlock), 0, st, pt, dt.dw, 2,
                        dt.dwin);
   if (op == POYHIP_OP_ALIGN_AFFINE) {
     AffFillArgs A;
