@@ -81,3 +81,32 @@ def test_set_tuning_rejects_bad_values(ctx):
     with pytest.raises(PoyhipError):
         ctx.chk(ctx._lib.poyhip_set_tuning(ctx.h, 99, 1))
     assert ctx._lib.poyhip_get_tuning(ctx.h, 99) < 0
+
+
+@pytest.mark.parametrize("n", [300, 700, 1500, 5000])
+def test_linear_batch_sizes_match_oracle(ctx, n):
+    # the linear fill follows the batch size (a workgroup per pair, or one
+    # wavefront per pair from 16 pairs per CU): sizes on both sides against
+    # the oracle's Align.align_2
+    import oracle
+    print(f"test_linear_batch_sizes_match_oracle n={n}", flush=True)
+    ctx.set_cm(cost_matrix.of_transformations_and_gaps(1, 2))
+    ocm = oracle.cm_dna(1, 2)
+    rng = np.random.default_rng(n)
+    plain = np.array([1, 2, 4, 8], np.uint8)
+    pairs = []
+    for _ in range(n):
+        root = rng.choice(plain, int(rng.integers(10, 120)))
+        sides = []
+        for _ in range(2):
+            s = root.copy()
+            s[rng.random(len(s)) < 0.1] = rng.choice(plain)
+            s = s[rng.random(len(s)) > 0.03]
+            sides.append(np.concatenate([[16], s]).astype(np.uint8))
+        pairs.append(tuple(sides))
+    costs, outs = run_pairs(ctx, "align_2d", pairs)
+    for k in range(0, n, max(1, n // 200)):
+        a, b = pairs[k]
+        p1, p2, c, _ = oracle.align_2(a, b, ocm)
+        assert int(costs[k]) == c, k
+        assert np.array_equal(outs[k][0], p1) and np.array_equal(outs[k][1], p2), k
