@@ -79,6 +79,14 @@ __device__ __forceinline__ bool union_merge_one(const uint8_t* a, const uint8_t*
   // begin cursors stay put; a union holds items while begin <= end
   const int abeg = au.len == 0 ? au.cap - 1 : au.cap - au.len;
   const int bbeg = bu.len == 0 ? bu.cap - 1 : bu.cap - bu.len;
+  // the column's symbols do not depend on the cursors: each iteration loads
+  // the next column's while it works on its own
+  int cm = 0, ca = 0, cb = 0;
+  if (i >= 0) {
+    cm = median[i];
+    ca = a[i];
+    cb = b[i];
+  }
   while (abeg <= au.end || bbeg <= bu.end) {
     if (c.end < 0 || c.pca < 0 || c.pcb < 0) return false;
     // union_copy_non_homologous (src/union.c:118): the count at each
@@ -96,7 +104,13 @@ __device__ __forceinline__ bool union_merge_one(const uint8_t* a, const uint8_t*
         if (!un_item(bu, c, gap)) return false;
     }
     if (i < 0) return false;  // "infinit loop" in the reference
-    int interm = median[i];
+    int interm = cm;
+    const int ai = ca, bi = cb;
+    if (i > 0) {
+      cm = median[i - 1];
+      ca = a[i - 1];
+      cb = b[i - 1];
+    }
     if (interm == gap) {
       if (i == 0) {
         if (!un_fill(c, gap, items, 0, 0)) return false;
@@ -105,8 +119,8 @@ __device__ __forceinline__ bool union_merge_one(const uint8_t* a, const uint8_t*
         return true;
       }
       bool ok;
-      if (a[i] == gap) ok = un_item(bu, c, gap);
-      else if (b[i] == gap) ok = un_item(au, c, gap);
+      if (ai == gap) ok = un_item(bu, c, gap);
+      else if (bi == gap) ok = un_item(au, c, gap);
       else {
         // union_prepend_pair (src/union.c:38)
         int v = gap;
@@ -128,7 +142,7 @@ __device__ __forceinline__ bool union_merge_one(const uint8_t* a, const uint8_t*
       int apos, bpos;
       // union_move_left (src/union.c:161) as the three cases call it; in
       // the last case each guard tests the other union's cursor, as written
-      if (a[i] == gap) {
+      if (ai == gap) {
         apos = 0;
         bpos = bu.position;
         if (bpos >= 0) {
@@ -136,7 +150,7 @@ __device__ __forceinline__ bool union_merge_one(const uint8_t* a, const uint8_t*
           bu.position--;
           bu.end--;
         }
-      } else if (b[i] == gap) {
+      } else if (bi == gap) {
         bpos = 0;
         apos = au.position;
         if (apos >= 0) {
