@@ -179,24 +179,28 @@ class _Arena:
         return at
 
     def put_all(self, hosts):
-        """Places host sequences in one transfer, each at a 16-byte boundary;
+        """Places host sequences, each at a 16-byte boundary: one transfer of
+        the bytes back to back, spread to their places on the device;
         returns [(at, len), ...]."""
         base = self.take(0)
-        ln = np.array([len(h) for h in hosts], np.int64)
+        ln = np.fromiter((len(h) for h in hosts), np.int64, len(hosts))
         size = (ln + 15) & ~15
         rel = np.zeros(len(hosts), np.int64)
+        starts = np.zeros(len(hosts), np.int64)
         if len(hosts) > 1:
             np.cumsum(size[:-1], out=rel[1:])
+            np.cumsum(ln[:-1], out=starts[1:])
         o = int(size.sum())
         self.take(o)
-        if o:
-            parts = []
-            for h, n, sz in zip(hosts, ln, size):
-                parts.append(np.asarray(h, np.uint8))
-                if sz > n:
-                    parts.append(np.zeros(int(sz - n), np.uint8))
-            self.t[base:base + o].copy_(torch.from_numpy(np.concatenate(parts)))
-        return [(base + int(r), int(n)) for r, n in zip(rel, ln)]
+        total = int(ln.sum())
+        if total:
+            dev = self.t.device
+            flat = torch.from_numpy(np.concatenate(hosts).astype(np.uint8, copy=False)).to(dev)
+            shift = torch.from_numpy(base + rel - starts).to(dev)
+            idx = torch.repeat_interleave(shift, torch.from_numpy(ln).to(dev), output_size=total)
+            idx += torch.arange(total, dtype=torch.int64, device=dev)
+            self.t[idx] = flat
+        return list(zip((base + rel).tolist(), ln.tolist()))
 
     def fetch(self, at, ln):
         """Host copies of the regions (offsets at, lengths ln): the bytes are
@@ -333,14 +337,13 @@ def readjust_level(ctx: Context, triples, use_ukk: bool = False, device="cuda"):
     run = np.concatenate([[0], np.cumsum(flat != GAP)])
     empty = run[st + s2_ln] - run[st] == 0
     new_at, new_ln = s2_at.copy(), s2_ln.copy()
-    eq, al = [], []
-    for k in np.nonzero(~empty)[0]:
-        h1 = seqs[k][third[pick[k]]]  # the chosen third sequence
-        if len(h1) == s2_ln[k] and np.array_equal(h1, flat[st[k]:st[k] + s2_ln[k]]):
-            eq.append(k)
-        else:
-            al.append(k)
-    eq, al = np.array(eq, np.int64), np.array(al, np.int64)
+    cand = np.nonzero(~empty)[0]
+    # only a chosen third sequence of the median's length can equal it
+    same = ln[cand, third[pick[cand]]] == s2_ln[cand]
+    eq = np.array([k for k in cand[same]
+                   if np.array_equal(seqs[k][third[pick[k]]], flat[st[k]:st[k] + s2_ln[k]])],
+                  np.int64)
+    al = np.setdiff1d(cand, eq)
     if len(eq):
         new_at[eq], new_ln[eq] = _post(ctx, arena, _lib.POST_CLOSEST_EQ, s1_at[eq], s2_at[eq],
                                        s1_ln[eq])
