@@ -71,7 +71,8 @@ const char *poyhip_last_error(const poyhip_ctx *ctx);
 /* cm_CAML_create + cm_CAML_set_* (src/cm.c:2452, 2345-2407): installs the
  * 2D cost model used by every later call on the context. */
 int poyhip_set_cm(poyhip_ctx *ctx, const poyhip_cm *cm);
-/* mat_CAML_flush_memory (src/matrices.c:352): frees the context scratch. */
+/* mat_CAML_flush_memory (src/matrices.c:352): frees the context scratch,
+ * and the device blocks destroyed batches left to the context. */
 int poyhip_flush(poyhip_ctx *ctx);
 
 /* ---- per-pair drop-ins (one launch per pair, synchronous) --------------- */
@@ -287,6 +288,10 @@ typedef struct poyhip_batch poyhip_batch;
 int poyhip_batch_create(poyhip_ctx *ctx, int op, int n, const int *len1,
                         const int *len2, const int *deltaw, size_t dm_budget_bytes,
                         poyhip_batch **out);
+/* Waits for the context's stream and hands the batch's device blocks to
+ * the context, which gives them to its next batches (up to 4 GiB in all;
+ * poyhip_flush and poyhip_destroy free them).  Destroy every batch before
+ * its context. */
 int poyhip_batch_destroy(poyhip_batch *b);
 /* Bytes of the output block of the batch and the offset of pair k in it.
  * Align ops write four consecutive regions of cap_k bytes per pair (affine:

@@ -48,6 +48,18 @@ struct DevBuf {
   size_t cap = 0;
 };
 
+// device blocks that destroyed batches of a context leave for its next
+// batches.  Batches of one context run in order on its stream and their
+// destroy waits for that stream, so a block handed on is idle.  A block
+// serves a request of at least half its size; kSpareBytes bound what is kept
+struct SpareBlocks {
+  static constexpr size_t kSpareBytes = size_t(4) << 30;
+  static constexpr size_t kMaxBlocks = 64;
+  std::mutex m;
+  std::vector<std::pair<void*, size_t>> idle, live;
+  size_t idle_bytes = 0;
+};
+
 }  // namespace
 
 struct poyhip_ctx {
@@ -104,9 +116,62 @@ struct poyhip_ctx {
   // it up; null when the lookup failed, with the reason in scratch_err)
   uint8_t* scratch = nullptr;
   char scratch_err[160] = {0};
+  SpareBlocks spare;
 };
 
 namespace {
+
+// a batch's device block: an idle one of the context when one fits, else
+// hipMalloc
+hipError_t ctx_malloc(poyhip_ctx* c, void** p, size_t n) {
+  SpareBlocks& sp = c->spare;
+  std::lock_guard<std::mutex> g(sp.m);
+  int best = -1;
+  for (size_t k = 0; k < sp.idle.size(); k++) {
+    const size_t sz = sp.idle[k].second;
+    if (sz >= n && sz / 2 <= n && (best < 0 || sz < sp.idle[best].second)) best = (int)k;
+  }
+  if (best >= 0) {
+    *p = sp.idle[best].first;
+    sp.live.push_back(sp.idle[best]);
+    sp.idle_bytes -= sp.idle[best].second;
+    sp.idle.erase(sp.idle.begin() + best);
+    return hipSuccess;
+  }
+  const hipError_t e = hipMalloc(p, n);
+  if (e == hipSuccess && *p) sp.live.push_back({*p, n});
+  return e;
+}
+
+// gives a batch's block back to the context (the caller has waited for the
+// context's stream)
+void ctx_release(poyhip_ctx* c, void* p) {
+  if (!p) return;
+  SpareBlocks& sp = c->spare;
+  std::lock_guard<std::mutex> g(sp.m);
+  size_t sz = 0;
+  for (size_t k = 0; k < sp.live.size(); k++)
+    if (sp.live[k].first == p) {
+      sz = sp.live[k].second;
+      sp.live.erase(sp.live.begin() + k);
+      break;
+    }
+  if (sz && sz <= SpareBlocks::kSpareBytes - sp.idle_bytes &&
+      sp.idle.size() < SpareBlocks::kMaxBlocks) {
+    sp.idle.push_back({p, sz});
+    sp.idle_bytes += sz;
+    return;
+  }
+  (void)hipFree(p);
+}
+
+void ctx_free_idle(poyhip_ctx* c) {
+  SpareBlocks& sp = c->spare;
+  std::lock_guard<std::mutex> g(sp.m);
+  for (auto& blk : sp.idle) (void)hipFree(blk.first);
+  sp.idle.clear();
+  sp.idle_bytes = 0;
+}
 
 int set_err(poyhip_ctx* c, int code, const char* fmt, ...) {
   if (c) {
@@ -511,6 +576,7 @@ int poyhip_flush(poyhip_ctx* c) {
     b->p = nullptr;
     b->cap = 0;
   }
+  ctx_free_idle(c);
   c->last2d = false;
   c->last3d = false;
   return POYHIP_OK;
@@ -1213,7 +1279,7 @@ int64_t lin_dm_exact(int L1, int L2, int newk, bool full) {
 void batch_free_device(poyhip_batch* b) {
   for (void* p : {(void*)b->d_tab, (void*)b->d_res, (void*)b->d_dm, (void*)b->d_ws,
                   (void*)b->d_stab, (void*)b->d_counters, (void*)b->d_cls})
-    if (p) (void)hipFree(p);
+    ctx_release(b->ctx, p);
   b->d_cls = nullptr;
   b->d_tab = nullptr;
   b->d_res = nullptr;
@@ -1481,7 +1547,7 @@ int poyhip_batch_create(poyhip_ctx* c, int op, int n, const int* len1, const int
     delete b;
     return set_err(c, code, "batch allocation failed: %s", what);
   };
-  if (hipMalloc(&b->d_tab, tab_bytes) != hipSuccess) return fail(POYHIP_ENOMEM, "tables");
+  if (ctx_malloc(c, (void**)&b->d_tab, tab_bytes) != hipSuccess) return fail(POYHIP_ENOMEM, "tables");
   char* p = static_cast<char*>(b->d_tab);
   b->d_dm_off = reinterpret_cast<int64_t*>(p); p += 8 * (size_t)n;
   b->d_stab_off = reinterpret_cast<int64_t*>(p); p += 8 * (size_t)n;
@@ -1492,19 +1558,19 @@ int poyhip_batch_create(poyhip_ctx* c, int op, int n, const int* len1, const int
   b->d_dw = reinterpret_cast<int*>(p); p += 4 * (size_t)n;
   b->d_dwin = reinterpret_cast<int*>(p); p += 4 * (size_t)n;
   b->d_order = reinterpret_cast<int*>(p); p += 4 * (size_t)n;
-  if (hipMalloc(&b->d_res, sizeof(PairResult) * (size_t)n) != hipSuccess)
+  if (ctx_malloc(c, (void**)&b->d_res, sizeof(PairResult) * (size_t)n) != hipSuccess)
     return fail(POYHIP_ENOMEM, "results");
   const size_t ncounters = b->two_phase ? 2 * (size_t)n + 4 : 2 * b->chunks.size() + 2;
-  if (hipMalloc(&b->d_counters, sizeof(int) * ncounters) != hipSuccess)
+  if (ctx_malloc(c, (void**)&b->d_counters, sizeof(int) * ncounters) != hipSuccess)
     return fail(POYHIP_ENOMEM, "counters");
-  if (hipMalloc(&b->d_cls, (size_t)n + 64) != hipSuccess) return fail(POYHIP_ENOMEM, "classes");
+  if (ctx_malloc(c, (void**)&b->d_cls, (size_t)n + 64) != hipSuccess) return fail(POYHIP_ENOMEM, "classes");
   if (op_has_dm(op)) {
-    if (hipMalloc(&b->d_dm, (size_t)b->dm_cap + 256) != hipSuccess)
+    if (ctx_malloc(c, (void**)&b->d_dm, (size_t)b->dm_cap + 256) != hipSuccess)
       return fail(POYHIP_ENOMEM, "direction store");
-    if (hipMalloc(&b->d_stab, sizeof(long long) * (size_t)(so + 1)) != hipSuccess)
+    if (ctx_malloc(c, (void**)&b->d_stab, sizeof(long long) * (size_t)(so + 1)) != hipSuccess)
       return fail(POYHIP_ENOMEM, "stripe tables");
   }
-  if (hipMalloc(&b->d_ws, sizeof(int) * (size_t)b->ws_stride * b->grid) != hipSuccess)
+  if (ctx_malloc(c, (void**)&b->d_ws, sizeof(int) * (size_t)b->ws_stride * b->grid) != hipSuccess)
     return fail(POYHIP_ENOMEM, "workspace");
   std::vector<int> dwin(n, 0);
   if (deltaw) {
@@ -2076,7 +2142,7 @@ namespace {
 
 void batch3_free(poyhip_batch3* b) {
   for (void* p : {(void*)b->d_tab, (void*)b->d_counters, (void*)b->d_dm, (void*)b->d_ring})
-    if (p) (void)hipFree(p);
+    ctx_release(b->ctx, p);
   b->d_tab = nullptr;
   b->d_counters = nullptr;
   b->d_dm = nullptr;
@@ -2184,7 +2250,7 @@ int poyhip_batch3_create(poyhip_ctx* c, int op, int n, const int* len1, const in
     return set_err(c, code, "triple batch allocation failed: %s", what);
   };
   const size_t tab_bytes = (size_t)n * (2 * 8 + 4 * 4) + 1024;
-  if (hipMalloc(&b->d_tab, tab_bytes) != hipSuccess) return fail(POYHIP_ENOMEM, "tables");
+  if (ctx_malloc(c, (void**)&b->d_tab, tab_bytes) != hipSuccess) return fail(POYHIP_ENOMEM, "tables");
   char* p = static_cast<char*>(b->d_tab);
   b->d_dm_off = reinterpret_cast<int64_t*>(p); p += 8 * (size_t)n;
   b->d_out_off = reinterpret_cast<int64_t*>(p); p += 8 * (size_t)n;
@@ -2192,11 +2258,11 @@ int poyhip_batch3_create(poyhip_ctx* c, int op, int n, const int* len1, const in
   b->d_l2 = reinterpret_cast<int*>(p); p += 4 * (size_t)n;
   b->d_l3 = reinterpret_cast<int*>(p); p += 4 * (size_t)n;
   b->d_order = reinterpret_cast<int*>(p); p += 4 * (size_t)n;
-  if (hipMalloc(&b->d_counters, sizeof(int) * (2 * b->chunks.size() + 2)) != hipSuccess)
+  if (ctx_malloc(c, (void**)&b->d_counters, sizeof(int) * (2 * b->chunks.size() + 2)) != hipSuccess)
     return fail(POYHIP_ENOMEM, "counters");
-  if (op == POYHIP_OP_ALIGN_3D && hipMalloc(&b->d_dm, (size_t)b->dm_cap + 256) != hipSuccess)
+  if (op == POYHIP_OP_ALIGN_3D && ctx_malloc(c, (void**)&b->d_dm, (size_t)b->dm_cap + 256) != hipSuccess)
     return fail(POYHIP_ENOMEM, "direction store");
-  if (hipMalloc(&b->d_ring, sizeof(int) * (size_t)b->ring_stride * b->grid) != hipSuccess)
+  if (ctx_malloc(c, (void**)&b->d_ring, sizeof(int) * (size_t)b->ring_stride * b->grid) != hipSuccess)
     return fail(POYHIP_ENOMEM, "rings");
   hipStream_t st = c->stream;
   bool ok = true;

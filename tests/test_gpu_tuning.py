@@ -110,3 +110,31 @@ def test_linear_batch_sizes_match_oracle(ctx, n):
         p1, p2, c, _ = oracle.align_2(a, b, ocm)
         assert int(costs[k]) == c, k
         assert np.array_equal(outs[k][0], p1) and np.array_equal(outs[k][1], p2), k
+
+
+
+def test_batches_reuse_device_blocks(ctx):
+    # a destroyed batch's device blocks go to the context's next batches with
+    # whatever they last held: a large affine batch, then a smaller one that
+    # takes its blocks, then one after poyhip_flush freed them, each against
+    # the oracle's align_affine_3
+    import oracle
+    print("test_batches_reuse_device_blocks", flush=True)
+    ctx.set_cm(cost_matrix.of_transformations_and_gaps(1, 1, 3))
+    ocm = oracle.cm_dna(1, 1, 3)
+    pairs = [p for p in workloads.pairs_list(2, 900, seed=0xB10C)
+             if len(p[0]) * len(p[1]) < 1_200_000]
+    for label, part in (("large", pairs), ("reused", pairs[:150]), ("flushed", pairs[150:300])):
+        if label == "flushed":
+            ctx.chk(ctx._lib.poyhip_flush(ctx.h))
+        print(f"  {label}: {len(part)} pairs", flush=True)
+        costs, outs = run_pairs(ctx, "align_affine", part)
+        for k in range(0, len(part), 9):
+            a, b = part[k]
+            sw = len(a) > len(b)
+            si, sj = (b, a) if sw else (a, b)
+            ref = oracle.align_affine_3(si, sj, ocm, int(sw))
+            ra, rb = (ref["resj"], ref["resi"]) if sw else (ref["resi"], ref["resj"])
+            assert costs[k] == ref["cost"], (label, k)
+            assert np.array_equal(outs[k][0], ra) and np.array_equal(outs[k][1], rb), (label, k)
+            assert np.array_equal(outs[k][2], ref["median"]), (label, k)
