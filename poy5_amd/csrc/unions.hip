@@ -49,6 +49,16 @@ __device__ __forceinline__ bool un_item(UnCursor& src, UnOut& c, int v) {
   return un_prepend(c, v);
 }
 
+// union_prepend_item with the symbol at src's position already loaded (sym)
+__device__ __forceinline__ bool un_item_at(UnCursor& src, UnOut& c, int v, int sym) {
+  if (src.position >= 0) {
+    v |= sym;
+    src.position--;
+    src.end--;
+  }
+  return un_prepend(c, v);
+}
+
 // union_prepend_and_fill_items (src/union.c:136): the counts prep..1 of the
 // items prepended since the last fill, their child offsets 0, then the
 // column's symbol with the children's positions (the new union is cleared
@@ -89,19 +99,26 @@ __device__ __forceinline__ bool union_merge_one(const uint8_t* a, const uint8_t*
   }
   while (abeg <= au.end || bbeg <= bu.end) {
     if (c.end < 0 || c.pca < 0 || c.pcb < 0) return false;
+    // the counts at both cursors and the symbols at both positions load
+    // together; a symbol is loaded again when items moved its cursor
+    const bool pa = au.position > -1, pb = bu.position > -1;
+    const int ka = pa ? aoff[au.end] : 0;
+    const int kb = pb ? boff[bu.end] : 0;
+    int sa = pa ? un_get(au, au.position) : 0;
+    int sb = pb ? un_get(bu, bu.position) : 0;
     // union_copy_non_homologous (src/union.c:118): the count at each
     // child's cursor is the number of its items copied across as they are
-    if (au.position > -1) {
-      int k = aoff[au.end];
-      items += k;
-      for (; k > 0; k--)
+    items += ka;
+    if (ka > 0) {
+      for (int k = ka; k > 0; k--)
         if (!un_item(au, c, gap)) return false;
+      if (au.position >= 0) sa = un_get(au, au.position);
     }
-    if (bu.position > -1) {
-      int k = boff[bu.end];
-      items += k;
-      for (; k > 0; k--)
+    items += kb;
+    if (kb > 0) {
+      for (int k = kb; k > 0; k--)
         if (!un_item(bu, c, gap)) return false;
+      if (bu.position >= 0) sb = un_get(bu, bu.position);
     }
     if (i < 0) return false;  // "infinit loop" in the reference
     int interm = cm;
@@ -119,18 +136,18 @@ __device__ __forceinline__ bool union_merge_one(const uint8_t* a, const uint8_t*
         return true;
       }
       bool ok;
-      if (ai == gap) ok = un_item(bu, c, gap);
-      else if (bi == gap) ok = un_item(au, c, gap);
+      if (ai == gap) ok = un_item_at(bu, c, gap, sb);
+      else if (bi == gap) ok = un_item_at(au, c, gap, sa);
       else {
         // union_prepend_pair (src/union.c:38)
         int v = gap;
         if (au.position >= 0) {
-          v |= un_get(au, au.position);
+          v |= sa;
           au.position--;
           au.end--;
         }
         if (bu.position >= 0) {
-          v |= un_get(bu, bu.position);
+          v |= sb;
           bu.position--;
           bu.end--;
         }
@@ -146,7 +163,7 @@ __device__ __forceinline__ bool union_merge_one(const uint8_t* a, const uint8_t*
         apos = 0;
         bpos = bu.position;
         if (bpos >= 0) {
-          interm |= un_get(bu, bu.position);
+          interm |= sb;
           bu.position--;
           bu.end--;
         }
@@ -154,20 +171,22 @@ __device__ __forceinline__ bool union_merge_one(const uint8_t* a, const uint8_t*
         bpos = 0;
         apos = au.position;
         if (apos >= 0) {
-          interm |= un_get(au, au.position);
+          interm |= sa;
           au.position--;
           au.end--;
         }
       } else {
         bpos = bu.position;
         apos = au.position;
+        // a guard that passes with the union's own cursor below 0 reads
+        // the byte before its live part, as the reference does
         if (bpos >= 0) {
-          interm |= un_get(au, au.position);
+          interm |= apos >= 0 ? sa : un_get(au, au.position);
           au.position--;
           au.end--;
         }
         if (apos >= 0) {
-          interm |= un_get(bu, bu.position);
+          interm |= bpos >= 0 ? sb : un_get(bu, bu.position);
           bu.position--;
           bu.end--;
         }
